@@ -1,0 +1,560 @@
+// C ABI of the SOUFFLE GPU engine: device memory, parameter tables, the step driver with
+// list-capacity recovery, energies and state transfer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "sf_internal.hpp"
+
+using namespace sf;
+
+namespace {
+
+#define HIPCHK(x)                                                                      \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) {                                                            \
+      set_error(std::string(#x) + ": " + hipGetErrorString(e_));                       \
+      return SF_E_HIP;                                                                 \
+    }                                                                                  \
+  } while (0)
+
+constexpr int kChunk = 100;          // steps between capacity checks
+constexpr int kTimeEvery = 10;       // with timing on, every 10th step's kernels get HIP events
+constexpr double kBoundMargin = 1.0; // initial slack of the centre-bead radius (sigma)
+
+}  // namespace
+
+struct sf_handle {
+  sf_template t;
+  int R = 0, Ns = 0, Nb = 0, Nq = 0;
+  std::vector<int> ns;
+  std::vector<sf_params> prm;
+  std::vector<RepParams> rp;
+  hipStream_t st = nullptr;
+  std::vector<void*> allocs;
+  Dev d{};
+  double margin = kBoundMargin;
+  int mode = MODE_MD;
+  int fp32 = 0;
+  bool timing = false;
+  double t_int = 0, t_force = 0, t_red = 0, t_steps = 0;
+  // snapshot for replaying a chunk
+  double4* s_pos = nullptr; double* s_v = nullptr; double* s_f = nullptr; double* s_g = nullptr;
+  RepState* s_rs = nullptr;
+  double* out_buf = nullptr;  // 5 x Nb x 3 for sf_forces
+  double* tab = nullptr;      // per-replica qq / yk / bd_amp tables
+
+  template <typename T>
+  int alloc(T** p, size_t n) {
+    void* q = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&q, n * sizeof(T)));
+    allocs.push_back(q);
+    *p = (T*)q;
+    return SF_OK;
+  }
+  void release(void* p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) { allocs.erase(it); (void)hipFree(p); }
+  }
+  ~sf_handle() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  void fill_bounds() {
+    const double rb = t.r_all + margin, rq = t.r_q + margin;
+    for (int r = 0; r < R; r++) {
+      RepParams& P = rp[r];
+      const sf_params& p = prm[r];
+      // largest LJ/WCA list radius over the template's bead pairs
+      double smax = 0;
+      for (int l = 0; l < t.np; l++) smax = std::max(smax, t.sigma[l]);
+      bool any_att = false;
+      for (int l = 0; l < t.np; l++)
+        if (t.att_mask[t.type[l]]) any_att = true;
+      double rlj = kWcaFactor * smax;
+      if (any_att) rlj = std::max(rlj, p.ecut);
+      rlj += p.skin;
+      P.sub_rc_lj2 = (rlj + 2 * rb) * (rlj + 2 * rb);
+      const double rql = p.ecut_el + p.skin;
+      P.sub_rc_q2 = (rql + 2 * rq) * (rql + 2 * rq);
+      P.r_bound2 = rb * rb;
+    }
+  }
+
+  int upload_rp() {
+    HIPCHK(hipMemcpyAsync((void*)d.rp, rp.data(), R * sizeof(RepParams), hipMemcpyHostToDevice, st));
+    return SF_OK;
+  }
+
+  int grow_lists(int need_lj, int need_q) {
+    if (need_lj > d.lj_cap) {
+      int cap = std::max(need_lj + need_lj / 4 + 8, d.lj_cap * 2);
+      release(d.lj_list);
+      int* p;
+      if (alloc(&p, (size_t)cap * Nb)) return SF_E_HIP;
+      d.lj_list = p; d.lj_cap = cap;
+    }
+    if (need_q > d.q_cap) {
+      int cap = std::max(need_q + need_q / 4 + 8, d.q_cap * 2);
+      release(d.q_list);
+      int* p;
+      if (alloc(&p, (size_t)cap * std::max(Nq, 1))) return SF_E_HIP;
+      d.q_list = p; d.q_cap = cap;
+    }
+    return SF_OK;
+  }
+
+  int read_states(std::vector<RepState>& h) {
+    h.resize(R);
+    HIPCHK(hipMemcpyAsync(h.data(), d.rs, R * sizeof(RepState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return SF_OK;
+  }
+
+  // After a launch sequence: inspect overflow / deformation flags. Returns 1 if the sequence
+  // must be replayed (capacities or bounds were raised), 0 if fine, <0 on error.
+  int check_and_fix() {
+    std::vector<RepState> h;
+    if (int rc = read_states(h)) return rc;
+    int need_lj = 0, need_q = 0, need_c = 0, deformed = 0;
+    for (auto& s : h) {
+      need_lj = std::max(need_lj, s.need_lj);
+      need_q = std::max(need_q, s.need_q);
+      need_c = std::max(need_c, s.need_cand);
+      deformed |= s.deformed;
+    }
+    if (!need_lj && !need_q && !need_c && !deformed) return 0;
+    if (need_c) {
+      set_error("more than 1024 candidate subunits near one subunit at a list build");
+      return SF_E_OVERFLOW;
+    }
+    if (deformed) {
+      margin += 1.0;
+      if (margin > 20.0) { set_error("subunit deformed beyond 20 sigma of its centre bead"); return SF_E_OVERFLOW; }
+      fill_bounds();
+      if (int rc = upload_rp()) return rc;
+    }
+    if (int rc = grow_lists(need_lj, need_q)) return rc;
+    return 1;
+  }
+
+  int clear_flags_force_rebuild() {
+    std::vector<RepState> h;
+    if (int rc = read_states(h)) return rc;
+    for (auto& s : h) { s.need_lj = s.need_q = s.need_cand = s.deformed = 0; s.rebuild = 1; }
+    HIPCHK(hipMemcpyAsync(d.rs, h.data(), R * sizeof(RepState), hipMemcpyHostToDevice, st));
+    return SF_OK;
+  }
+
+  int snapshot(bool save) {
+    const size_t nb = Nb;
+    hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    if (save) {
+      HIPCHK(hipMemcpyAsync(s_pos, d.pos, nb * sizeof(double4), k, st));
+      HIPCHK(hipMemcpyAsync(s_v, d.vx, 3 * nb * sizeof(double), k, st));
+      HIPCHK(hipMemcpyAsync(s_f, d.fx, 3 * nb * sizeof(double), k, st));
+      HIPCHK(hipMemcpyAsync(s_g, d.gx, 3 * nb * sizeof(double), k, st));
+      HIPCHK(hipMemcpyAsync(s_rs, d.rs, R * sizeof(RepState), k, st));
+    } else {
+      HIPCHK(hipMemcpyAsync(d.pos, s_pos, nb * sizeof(double4), k, st));
+      HIPCHK(hipMemcpyAsync(d.vx, s_v, 3 * nb * sizeof(double), k, st));
+      HIPCHK(hipMemcpyAsync(d.fx, s_f, 3 * nb * sizeof(double), k, st));
+      HIPCHK(hipMemcpyAsync(d.gx, s_g, 3 * nb * sizeof(double), k, st));
+      HIPCHK(hipMemcpyAsync(d.rs, s_rs, R * sizeof(RepState), k, st));
+      if (int rc = launch_centres(d, st)) return rc;
+    }
+    return SF_OK;
+  }
+
+  // start: forces with a list build, kinetic energy, thermostat and noise set-up
+  int start_sequence() {
+    for (int attempt = 0; attempt < 16; attempt++) {
+      if (int rc = launch_centres(d, st)) return rc;
+      if (int rc = launch_force(d, MODE_FORCES, 1, 1, fp32, st)) return rc;
+      if (int rc = launch_finalize(d, mode, 1, st)) return rc;
+      int rc = check_and_fix();
+      if (rc < 0) return rc;
+      if (rc == 0) return SF_OK;
+      std::vector<RepState> h;
+      if ((rc = read_states(h))) return rc;
+      for (auto& s : h) { s.need_lj = s.need_q = s.need_cand = s.deformed = 0; }
+      HIPCHK(hipMemcpyAsync(d.rs, h.data(), R * sizeof(RepState), hipMemcpyHostToDevice, st));
+    }
+    set_error("list capacity did not converge");
+    return SF_E_OVERFLOW;
+  }
+
+  int run_chunk(int n, std::vector<hipEvent_t>* ev) {
+    for (int k = 0; k < n; k++) {
+      const bool tm = ev && (k % kTimeEvery == 0);
+      hipEvent_t* e = tm ? &(*ev)[4 * (k / kTimeEvery)] : nullptr;
+      if (tm) HIPCHK(hipEventRecord(e[0], st));
+      if (int rc = launch_integrate(d, mode, st)) return rc;
+      if (tm) HIPCHK(hipEventRecord(e[1], st));
+      if (int rc = launch_force(d, mode, 0, 0, fp32, st)) return rc;
+      if (tm) HIPCHK(hipEventRecord(e[2], st));
+      if (int rc = launch_finalize(d, mode, 0, st)) return rc;
+      if (tm) HIPCHK(hipEventRecord(e[3], st));
+    }
+    return SF_OK;
+  }
+};
+
+extern "C" {
+
+sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const sf_params* params) {
+  if (!t || R <= 0 || !n_subunits || !params) { set_error("bad arguments to sf_create"); return nullptr; }
+  const int np = t->np, nq = t->nq();
+  if (np > kBlock) { set_error("template has more than 128 beads"); return nullptr; }
+  if ((size_t)(2 * np) * sizeof(double4) + (size_t)t->nh() * 12 * sizeof(double) > 60 * 1024) {
+    set_error("template too large for the subunit workgroup's LDS"); return nullptr;
+  }
+  int brown = params[0].brownian;
+  int fp32 = params[0].pair_fp32;
+  long long Ns = 0;
+  for (int r = 0; r < R; r++) {
+    const sf_params& p = params[r];
+    if (n_subunits[r] <= 0) { set_error("replica with no subunits"); return nullptr; }
+    if (p.brownian != brown || p.pair_fp32 != fp32) { set_error("all replicas must share the integrator and pair precision"); return nullptr; }
+    if (!(p.box > 0) || !(p.dt > 0) || !(p.skin > 0) || !(p.kappa > 0)) { set_error("box, dt, skin and kappa must be positive"); return nullptr; }
+    if (!(p.ecut_el > 0)) { set_error("uncut Yukawa (ecut_c = 0) is not supported by the list engine"); return nullptr; }
+    if (!brown && (p.q_chain < 1 || p.q_chain > kMaxChain)) { set_error("Nose-Hoover chain length must be 1..8"); return nullptr; }
+    if (brown && !(p.damp > 0)) { set_error("BD damping must be positive"); return nullptr; }
+    double reach = std::max(p.ecut, p.ecut_el) + p.skin;
+    if (reach > 0.5 * p.box) {
+      set_error("box too small for the cutoffs (minimum image needs cutoff + skin < box/2)"); return nullptr;
+    }
+    Ns += n_subunits[r];
+  }
+  if (Ns * np > 0x7fffffffLL / 2) { set_error("too many beads"); return nullptr; }
+  sf_handle* h = new sf_handle();
+  h->t = *t;
+  h->R = R;
+  h->Ns = (int)Ns;
+  h->Nb = (int)Ns * np;
+  h->Nq = (int)Ns * nq;
+  h->ns.assign(n_subunits, n_subunits + R);
+  h->prm.assign(params, params + R);
+  h->mode = brown ? MODE_BD : MODE_MD;
+  h->fp32 = fp32;
+  auto fail = [&](const char* what) -> sf_handle* {
+    if (what) set_error(what);
+    delete h;
+    return nullptr;
+  };
+  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed (no GPU?)");
+  Dev& d = h->d;
+  d.Nb = h->Nb; d.Ns = h->Ns; d.Nq = h->Nq; d.R = R;
+  // template tables
+  TopoDev& td = d.topo;
+  td.np = np; td.nh = t->nh(); td.nq = nq; td.centre = t->centre;
+  auto up = [&](auto** dst, const auto& v) -> int {
+    typedef typename std::remove_reference<decltype(v[0])>::type T0;
+    typedef typename std::remove_const<T0>::type T;
+    T* p;
+    if (h->alloc(&p, v.size())) return SF_E_HIP;
+    if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("hipMemcpy failed"); return SF_E_HIP;
+    }
+    *dst = p;
+    return SF_OK;
+  };
+  if (up((double**)&td.mass, t->mass) || up((double**)&td.sigma, t->sigma) || up((int**)&td.type, t->type) ||
+      up((int**)&td.qslot, t->qslot) || up((int**)&td.qlocal, t->qlocal) || up((int**)&td.e_off, t->e_off) ||
+      up((int**)&td.e_other, t->e_other) || up((double**)&td.e_len0, t->e_len0) ||
+      up((int**)&td.hinge, t->hinges) || up((int**)&td.h_off, t->h_off) || up((int**)&td.h_slot, t->h_slot) ||
+      up((unsigned long long**)&td.att_mask, t->att_mask))
+    return fail(nullptr);
+  // per-replica tables: qq [nq*nq], yk [nq*nq], bd_amp [np]
+  const size_t tstride = 2 * (size_t)nq * nq + np;
+  std::vector<double> tabs(tstride * R);
+  h->rp.assign(R, RepParams{});
+  std::vector<int> sub_rep(Ns);
+  int s0 = 0;
+  for (int r = 0; r < R; r++) {
+    const sf_params& p = params[r];
+    double* qq = &tabs[tstride * r];
+    double* yk = qq + (size_t)nq * nq;
+    double* amp = yk + (size_t)nq * nq;
+    for (int a = 0; a < nq; a++)
+      for (int b = 0; b < nq; b++) {
+        int la = t->qlocal[a], lb = t->qlocal[b];
+        double sig1 = t->sigma[la], sig2 = t->sigma[lb];
+        double shc = (sig1 + sig2) / 2;
+        qq[a * nq + b] = t->q[la] * t->q[lb] * p.lb;
+        yk[a * nq + b] = std::exp(p.kappa * shc) / ((1 + p.kappa * sig1 / 2) * (1 + p.kappa * sig2 / 2));
+      }
+    for (int l = 0; l < np; l++) amp[l] = brown ? std::sqrt((t->mass[l] * 24.0) / (p.damp * p.dt)) : 0.0;
+    RepParams& P = h->rp[r];
+    P.box = p.box; P.hbox = p.box * 0.5;
+    P.ecut = p.ecut; P.ecut2 = p.ecut * p.ecut;
+    P.ecut_el = p.ecut_el; P.ecut_el2 = p.ecut_el * p.ecut_el;
+    P.kappa = p.kappa; P.lb = p.lb; P.elj_att = p.elj_att;
+    P.ks = p.ks; P.kb = p.kb; P.dt = p.dt; P.damp = p.damp; P.T = p.T;
+    P.skin = p.skin; P.half_skin2 = 0.25 * p.skin * p.skin;
+    P.rc_q_list2 = (p.ecut_el + p.skin) * (p.ecut_el + p.skin);
+    P.s0 = s0; P.ns = n_subunits[r]; P.b0 = s0 * np; P.nb = n_subunits[r] * np;
+    P.q0 = s0 * nq; P.nqr = n_subunits[r] * nq;
+    P.inv_nb = 1.0 / (double)P.nb;
+    P.q_chain = brown ? 0 : p.q_chain;
+    P.brownian = brown; P.pair_fp32 = fp32;
+    P.seed = p.bd_seed % kLcgMod;
+    if (P.seed == 0) P.seed = 1;  // minstd_rand0 maps seed 0 to 1
+    P.mult6n = lcg_pow(6ull * (unsigned long long)P.nb);
+    for (int j = 0; j < kMaxChain; j++) { P.Qb[j] = 0; P.dof[j] = 0; }
+    if (!brown) {
+      const double dof3n = (double)(3ul * (unsigned long)P.nb);
+      if (p.q_chain == 1) { P.Qb[0] = 0; P.dof[0] = dof3n; }
+      else {
+        P.Qb[0] = p.Q; P.dof[0] = dof3n;
+        for (int j = 1; j < p.q_chain - 1; j++) { P.Qb[j] = 1; P.dof[j] = 1; }
+        P.Qb[p.q_chain - 1] = 0; P.dof[p.q_chain - 1] = dof3n;
+      }
+    }
+    for (int s = 0; s < n_subunits[r]; s++) sub_rep[s0 + s] = r;
+    s0 += n_subunits[r];
+  }
+  if (up(&h->tab, tabs)) return fail(nullptr);
+  for (int r = 0; r < R; r++) {
+    h->rp[r].qq = h->tab + tstride * r;
+    h->rp[r].yk = h->tab + tstride * r + (size_t)nq * nq;
+    h->rp[r].bd_amp = h->tab + tstride * r + 2 * (size_t)nq * nq;
+  }
+  h->fill_bounds();
+  RepParams* rpd;
+  RepState* rsd;
+  int* srd;
+  if (h->alloc(&rpd, R) || h->alloc(&rsd, R) || h->alloc(&srd, Ns)) return fail(nullptr);
+  d.rp = rpd; d.rs = rsd; d.sub_rep = srd;
+  if (hipMemcpy(srd, sub_rep.data(), Ns * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return fail("hipMemcpy failed");
+  if (hipMemcpy(rpd, h->rp.data(), R * sizeof(RepParams), hipMemcpyHostToDevice) != hipSuccess) return fail("hipMemcpy failed");
+  const size_t nb = h->Nb;
+  double* vbuf; double* fbuf; double* gbuf;
+  if (h->alloc(&d.pos, nb) || h->alloc(&vbuf, 3 * nb) || h->alloc(&fbuf, 3 * nb) || h->alloc(&gbuf, 3 * nb) ||
+      h->alloc(&d.xb, nb) || h->alloc(&d.centres, Ns) || h->alloc(&d.lj_cnt, nb) ||
+      h->alloc(&d.q_cnt, std::max(h->Nq, 1)) || h->alloc(&d.part, (size_t)Ns * kPartStride) ||
+      h->alloc(&d.epart, (size_t)Ns * 5) || h->alloc(&d.ebead, nb * 5) ||
+      h->alloc(&h->s_pos, nb) || h->alloc(&h->s_v, 3 * nb) || h->alloc(&h->s_f, 3 * nb) ||
+      h->alloc(&h->s_g, 3 * nb) || h->alloc(&h->s_rs, R))
+    return fail(nullptr);
+  d.vx = vbuf; d.vy = vbuf + nb; d.vz = vbuf + 2 * nb;
+  d.fx = fbuf; d.fy = fbuf + nb; d.fz = fbuf + 2 * nb;
+  d.gx = gbuf; d.gy = gbuf + nb; d.gz = gbuf + 2 * nb;
+  d.lj_cap = 0; d.q_cap = 0;
+  if (h->grow_lists(24, nq > 0 ? 48 : 1)) return fail(nullptr);
+  if (hipMemset(fbuf, 0, 3 * nb * sizeof(double)) != hipSuccess || hipMemset(gbuf, 0, 3 * nb * sizeof(double)) != hipSuccess ||
+      hipMemset(rsd, 0, R * sizeof(RepState)) != hipSuccess)
+    return fail("hipMemset failed");
+  return h;
+}
+
+void sf_destroy(sf_handle* h) { delete h; }
+
+int sf_set_state(sf_handle* h, const double* pos, const double* vel) {
+  if (!h || !pos) { set_error("null argument"); return SF_E_ARG; }
+  const int np = h->t.np;
+  const size_t nb = h->Nb;
+  std::vector<double4> hp(nb);
+  for (size_t i = 0; i < nb; i++) {
+    int s = (int)(i / np), l = (int)(i % np);
+    hp[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], make_tag(s, l));
+  }
+  std::vector<double> hv(3 * nb, 0.0);
+  if (vel)
+    for (size_t i = 0; i < nb; i++) {
+      hv[i] = vel[3 * i]; hv[nb + i] = vel[3 * i + 1]; hv[2 * nb + i] = vel[3 * i + 2];
+    }
+  HIPCHK(hipMemcpyAsync(h->d.pos, hp.data(), nb * sizeof(double4), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(h->d.vx, hv.data(), 3 * nb * sizeof(double), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemsetAsync(h->d.fx, 0, 3 * nb * sizeof(double), h->st));
+  HIPCHK(hipMemsetAsync(h->d.gx, 0, 3 * nb * sizeof(double), h->st));
+  std::vector<RepState> rs(h->R);
+  std::memset(rs.data(), 0, h->R * sizeof(RepState));
+  for (int r = 0; r < h->R; r++) { rs[r].rng = h->rp[r].seed; rs[r].expfac = 1.0; }
+  HIPCHK(hipMemcpyAsync(h->d.rs, rs.data(), h->R * sizeof(RepState), hipMemcpyHostToDevice, h->st));
+  int rc = h->start_sequence();
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SF_OK;
+}
+
+int sf_get_state(sf_handle* h, double* pos, double* vel) {
+  if (!h) { set_error("null handle"); return SF_E_ARG; }
+  const size_t nb = h->Nb;
+  if (pos) {
+    std::vector<double4> hp(nb);
+    HIPCHK(hipMemcpyAsync(hp.data(), h->d.pos, nb * sizeof(double4), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (size_t i = 0; i < nb; i++) { pos[3 * i] = hp[i].x; pos[3 * i + 1] = hp[i].y; pos[3 * i + 2] = hp[i].z; }
+  }
+  if (vel) {
+    std::vector<double> hv(3 * nb);
+    HIPCHK(hipMemcpyAsync(hv.data(), h->d.vx, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (size_t i = 0; i < nb; i++) { vel[3 * i] = hv[i]; vel[3 * i + 1] = hv[nb + i]; vel[3 * i + 2] = hv[2 * nb + i]; }
+  }
+  return SF_OK;
+}
+
+int sf_get_thermostat(sf_handle* h, int replica, double* xi, double* eta) {
+  if (!h || replica < 0 || replica >= h->R) { set_error("bad replica"); return SF_E_ARG; }
+  std::vector<RepState> s;
+  if (int rc = h->read_states(s)) return rc;
+  int q = h->rp[replica].q_chain;
+  for (int j = 0; j < q; j++) {
+    if (xi) xi[j] = s[replica].xi[j];
+    if (eta) eta[j] = s[replica].eta[j];
+  }
+  return q;
+}
+
+int sf_forces(sf_handle* h, double* f, double* fs, double* fb, double* fe, double* flj) {
+  if (!h) { set_error("null handle"); return SF_E_ARG; }
+  const size_t n3 = 3 * (size_t)h->Nb;
+  if (!h->out_buf && h->alloc(&h->out_buf, 5 * n3)) return SF_E_HIP;
+  Dev d = h->d;
+  d.out_f = h->out_buf; d.out_s = h->out_buf + n3; d.out_b = h->out_buf + 2 * n3;
+  d.out_e = h->out_buf + 3 * n3; d.out_lj = h->out_buf + 4 * n3;
+  for (int attempt = 0; attempt < 16; attempt++) {
+    d.lj_list = h->d.lj_list; d.q_list = h->d.q_list; d.lj_cap = h->d.lj_cap; d.q_cap = h->d.q_cap;
+    if (int rc = launch_centres(d, h->st)) return rc;
+    if (int rc = launch_force(d, MODE_FORCES, 0, 1, h->fp32, h->st)) return rc;
+    int rc = h->check_and_fix();
+    if (rc < 0) return rc;
+    if (rc == 0) break;
+    std::vector<RepState> s;
+    if ((rc = h->read_states(s))) return rc;
+    for (auto& x : s) { x.need_lj = x.need_q = x.need_cand = x.deformed = 0; }
+    HIPCHK(hipMemcpyAsync(h->d.rs, s.data(), h->R * sizeof(RepState), hipMemcpyHostToDevice, h->st));
+  }
+  double* outs[5] = {f, fs, fb, fe, flj};
+  for (int k = 0; k < 5; k++)
+    if (outs[k]) HIPCHK(hipMemcpyAsync(outs[k], h->out_buf + k * n3, n3 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return SF_OK;
+}
+
+int sf_energies(sf_handle* h, double* totals, double* per_bead) {
+  if (!h || !totals) { set_error("null argument"); return SF_E_ARG; }
+  if (int rc = launch_energy(h->d, h->st)) return rc;
+  std::vector<double> ep((size_t)h->Ns * 5);
+  HIPCHK(hipMemcpyAsync(ep.data(), h->d.epart, ep.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  if (per_bead)
+    HIPCHK(hipMemcpyAsync(per_bead, h->d.ebead, (size_t)h->Nb * 5 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  std::vector<RepState> s;
+  if (int rc = h->read_states(s)) return rc;
+  for (int r = 0; r < h->R; r++) {
+    const RepParams& P = h->rp[r];
+    double se = 0, be = 0, ne = 0, ce = 0, ke = 0;
+    for (int k = P.s0; k < P.s0 + P.ns; k++) {
+      se += ep[5 * k]; be += ep[5 * k + 1]; ne += ep[5 * k + 2]; ce += ep[5 * k + 3]; ke += ep[5 * k + 4];
+    }
+    double tp = 0, tk = 0;
+    for (int j = 0; j < P.q_chain; j++) {  // THERMOSTAT::potential_energy / kinetic_energy
+      tp += P.dof[j] * 1.0 * P.T * s[r].eta[j];
+      tk += 0.5 * P.Qb[j] * s[r].xi[j] * s[r].xi[j];
+    }
+    double* o = totals + 7 * r;
+    o[0] = ke; o[1] = se; o[2] = be; o[3] = ne; o[4] = ce; o[5] = tp; o[6] = tk;
+  }
+  return SF_OK;
+}
+
+int sf_step(sf_handle* h, int n_steps) {
+  if (!h || n_steps < 0) { set_error("bad arguments to sf_step"); return SF_E_ARG; }
+  std::vector<hipEvent_t> ev;
+  if (h->timing) {
+    ev.resize(4 * ((kChunk + kTimeEvery - 1) / kTimeEvery));
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  }
+  double ti = 0, tf = 0, tr = 0, nt = 0;
+  int done = 0, rc = SF_OK;
+  int retries = 0;
+  while (done < n_steps) {
+    const int n = std::min(kChunk, n_steps - done);
+    if ((rc = h->snapshot(true))) break;
+    if ((rc = h->run_chunk(n, h->timing ? &ev : nullptr))) break;
+    rc = h->check_and_fix();
+    if (rc < 0) break;
+    if (rc == 1) {
+      if (++retries > 16) { set_error("list capacity did not converge"); rc = SF_E_OVERFLOW; break; }
+      if ((rc = h->snapshot(false))) break;
+      if ((rc = h->clear_flags_force_rebuild())) break;
+      rc = SF_OK;
+      continue;
+    }
+    if (h->timing) {
+      for (int k = 0; k < n; k += kTimeEvery) {
+        const int i = k / kTimeEvery;
+        float a = 0, b = 0, c = 0;
+        (void)hipEventElapsedTime(&a, ev[4 * i], ev[4 * i + 1]);
+        (void)hipEventElapsedTime(&b, ev[4 * i + 1], ev[4 * i + 2]);
+        (void)hipEventElapsedTime(&c, ev[4 * i + 2], ev[4 * i + 3]);
+        ti += a; tf += b; tr += c;
+        nt += 1;
+      }
+    }
+    done += n;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc) return rc;
+  if ((rc = launch_apply_com(h->d, h->st))) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (h->timing && nt > 0) {
+    h->t_int = ti / nt; h->t_force = tf / nt; h->t_red = tr / nt; h->t_steps = nt;
+  }
+  return SF_OK;
+}
+
+int sf_set_timing(sf_handle* h, int enable) {
+  if (!h) { set_error("null handle"); return SF_E_ARG; }
+  h->timing = enable != 0;
+  return SF_OK;
+}
+
+int sf_get_stats(sf_handle* h, sf_stats* o) {
+  if (!h || !o) { set_error("null argument"); return SF_E_ARG; }
+  std::vector<RepState> s;
+  if (int rc = h->read_states(s)) return rc;
+  std::memset(o, 0, sizeof(*o));
+  o->steps = s[0].step;
+  for (auto& x : s) { o->rebuilds += x.rebuilds; o->lj_entries += x.lj_entries; o->q_entries += x.q_entries; }
+  o->lj_capacity = h->d.lj_cap; o->q_capacity = h->d.q_cap;
+  o->n_beads = h->Nb; o->n_subunits = h->Ns; o->n_charged = h->Nq; o->n_replicas = h->R;
+  o->k_integrate_ms = h->t_int; o->k_force_ms = h->t_force; o->k_reduce_ms = h->t_red; o->k_timed_steps = h->t_steps;
+  return SF_OK;
+}
+
+int sf_bd_uniforms(sf_handle* h, int replica, int64_t step, double* out) {
+  if (!h || replica < 0 || replica >= h->R || step < 0 || !out) { set_error("bad arguments"); return SF_E_ARG; }
+  const RepParams& P = h->rp[replica];
+  uint64_t e = 6ull * (uint64_t)P.nb;
+  // state before iteration `step`: seed * 16807^(6 nb step) mod (2^31-1)
+  uint64_t st = P.seed;
+  uint64_t mul = 1, base = lcg_pow(e);
+  for (uint64_t k = (uint64_t)step; k; k >>= 1) {
+    if (k & 1) mul = (mul * base) % kLcgMod;
+    base = (base * base) % kLcgMod;
+  }
+  st = (st * mul) % kLcgMod;
+  double* dout;
+  if (h->alloc(&dout, 3 * (size_t)P.nb)) return SF_E_HIP;
+  int rc = launch_bd_uniforms_state((uint32_t)st, P.nb, dout, h->d, h->st);
+  if (!rc) {
+    HIPCHK(hipMemcpyAsync(out, dout, 3 * (size_t)P.nb * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
+  h->release(dout);
+  return rc;
+}
+
+}  // extern "C"

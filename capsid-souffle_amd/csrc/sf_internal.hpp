@@ -1,0 +1,151 @@
+// Internal types shared by the host library and the HIP kernels.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "souffle_gpu.h"
+
+struct sf_template {
+  // header values (src/initialize.cpp:47)
+  double bondlength = 0, si_mass = 0, si_sigma = 0;
+  int np = 0, ne = 0, nf = 0;
+  // per template bead
+  std::vector<double> x, y, z, q, sigma, mass;
+  std::vector<int> type;
+  // edges and faces, template order
+  std::vector<int> e1, e2, etype;
+  std::vector<double> len0;
+  std::vector<int> f1, f2, f3;
+  // ordered attractive (type_i, type_j) rows of lj_a
+  std::vector<int> la1, la2;
+
+  // ---- derived once per template
+  int centre = 0;                 // bead closest to the template centroid
+  double r_all = 0, r_q = 0;      // max distance from the centre bead: all / charged beads
+  std::vector<int> hinges;        // [nh][4] B0, B1, across(F0), across(F1)
+  std::vector<int> e_off, e_other;  // per-bead incident edges (CSR, ascending edge id)
+  std::vector<double> e_len0;
+  std::vector<int> h_off, h_slot;   // per-bead hinge incidences (CSR), slot = 4 h + k
+  std::vector<int> qslot;           // charged index of each bead, -1 if uncharged
+  std::vector<int> qlocal;          // template index of each charged bead
+  std::vector<unsigned long long> att_mask;  // [64]: bit tj of att_mask[ti] = (ti,tj) attractive
+  int nh() const { return (int)hinges.size() / 4; }
+  int nq() const { return (int)qlocal.size(); }
+};
+
+namespace sf {
+
+constexpr int kBlock = 128;      // threads of a subunit workgroup; templates need np <= kBlock
+constexpr int kMaxChain = 8;     // Nose-Hoover chain length limit
+constexpr int kMaxTypes = 64;    // bead type ids must be < kMaxTypes
+constexpr int kMaxCand = 1024;   // candidate subunits per subunit at a list build
+constexpr int kPartStride = 10;  // per-subunit partial sums
+constexpr double kWcaFactor = 1.12246205;  // src/forces.cpp:27, src/energies.cpp:67
+constexpr uint32_t kLcgMod = 2147483647u;  // minstd_rand0: x <- 16807 x mod (2^31 - 1)
+constexpr uint32_t kLcgMul = 16807u;
+
+void set_error(const std::string& s);
+
+// Bead tags: subunit id in the high 32 bits, template-local index in the low 32 bits,
+// carried bit-for-bit in the w component of the position double4.
+inline double make_tag(int s, int l) {
+  unsigned long long u = ((unsigned long long)(unsigned)s << 32) | (unsigned)l;
+  double d;
+  __builtin_memcpy(&d, &u, 8);
+  return d;
+}
+
+// Device view of the template tables.
+struct TopoDev {
+  int np, nh, nq, centre;
+  const double* mass;
+  const double* sigma;
+  const int* type;
+  const int* qslot;
+  const int* qlocal;
+  const int* e_off;
+  const int* e_other;
+  const double* e_len0;
+  const int* hinge;   // [nh][4]
+  const int* h_off;
+  const int* h_slot;
+  const unsigned long long* att_mask;  // [kMaxTypes]
+};
+
+// Per-replica constants.
+struct RepParams {
+  double box, hbox;
+  double ecut, ecut2, ecut_el, ecut_el2, kappa, lb, elj_att;
+  double ks, kb, dt, damp, T;
+  double skin, half_skin2;
+  double sub_rc_lj2, sub_rc_q2;  // candidate-subunit centre distance^2 for a list build
+  double rc_q_list2;             // (ecut_el + skin)^2
+  double r_bound2;               // largest allowed bead distance^2 from its centre bead
+  double inv_nb;                 // 1.0 / beads of the replica
+  double Qb[kMaxChain];          // bath masses
+  double dof[kMaxChain];         // bath degrees of freedom (as double)
+  int s0, ns, b0, nb, q0, nqr;   // subunit / bead / charged-slot ranges
+  int q_chain, brownian, pair_fp32, pad;
+  uint32_t seed, mult6n;         // engine seed; 16807^(6 nb) mod (2^31-1)
+  const double* qq;              // [nq][nq] q_i q_j lb
+  const double* yk;              // [nq][nq] Yukawa size factor exp(k shc)/((1+k s_i/2)(1+k s_j/2))
+  const double* bd_amp;          // [np] sqrt(24 m / (damp dt))
+};
+
+// Per-replica dynamic state, device resident.
+struct RepState {
+  double xi[kMaxChain], eta[kMaxChain];      // after the last completed step
+  double xi_n[kMaxChain], eta_n[kMaxChain];  // after the current step's first thermostat half
+  double expfac;    // exp(-dt/2 xi_0) of the current step
+  double kick_c;    // 0.5 dt sqrt(expfac)
+  double ke;        // particle kinetic energy used by the chain
+  double mean_v[3];     // centre-of-mass velocity still to be removed
+  double mean_fran[3];  // mean BD noise of the current step
+  double vcom;          // |mean v| removed at the last step
+  uint32_t rng;         // engine state before the current step's draws
+  int rebuild;          // list rebuild requested (set by the integrate kernel)
+  int need_lj, need_q;  // longest list seen when it exceeded the capacity
+  int need_cand;        // candidate subunits when they exceeded kMaxCand
+  int deformed;         // a bead left the centre-bead bound at a list build
+  long long step, rebuilds, lj_entries, q_entries;
+};
+
+struct Dev {
+  TopoDev topo;
+  const RepParams* rp;
+  RepState* rs;
+  const int* sub_rep;
+  double4* pos;        // [Nb] x, y, z, tag
+  double* vx; double* vy; double* vz;
+  double* fx; double* fy; double* fz;  // integrator force (tforce)
+  double* gx; double* gy; double* gz;  // BD drag force
+  double4* xb;         // positions at the last list build
+  double4* centres;    // [Ns] centre-bead positions
+  int* lj_list; int* lj_cnt;
+  int* q_list; int* q_cnt;
+  double* part;        // [Ns][kPartStride]
+  double* epart;       // [Ns][5] energy partial sums
+  double* ebead;       // [Nb][5] per-bead energies
+  double* out_f; double* out_s; double* out_b; double* out_e; double* out_lj;  // [Nb][3], nullable
+  int lj_cap, q_cap;
+  int Nb, Ns, Nq, R;
+};
+
+enum { MODE_FORCES = 0, MODE_MD = 1, MODE_BD = 2 };
+
+// kernel launchers (kernels.hip)
+int launch_centres(const Dev& d, hipStream_t st);
+int launch_integrate(const Dev& d, int mode, hipStream_t st);
+int launch_force(const Dev& d, int mode, int init, int force_rebuild, int fp32, hipStream_t st);
+int launch_finalize(const Dev& d, int mode, int init, hipStream_t st);
+int launch_apply_com(const Dev& d, hipStream_t st);
+int launch_energy(const Dev& d, hipStream_t st);
+int launch_bd_uniforms_state(uint32_t state, int nb, double* out_dev, const Dev& d, hipStream_t st);
+
+// host helpers (topology.cpp)
+uint32_t lcg_pow(uint64_t e);   // 16807^e mod (2^31-1)
+bool derive_template(sf_template& t);
+
+}  // namespace sf

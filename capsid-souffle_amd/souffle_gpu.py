@@ -1,0 +1,218 @@
+"""ctypes binding of libsouffle_gpu.so (C ABI in include/souffle_gpu.h).
+
+Thin convenience layer for Python callers: it converts numpy arrays to the library's plain
+pointers and raises on error codes. There is no fallback: if the HIP library is not built or
+no GPU is present, loading or the first device call fails loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsouffle_gpu.so")
+REPO = os.path.dirname(_HERE)
+
+
+class SfCli(ctypes.Structure):
+    _fields_ = [("C_uM", ctypes.c_double), ("c_mM", ctypes.c_double), ("ks", ctypes.c_double),
+                ("kb", ctypes.c_double), ("elj_att", ctypes.c_double), ("ecut_c", ctypes.c_double),
+                ("dt", ctypes.c_double), ("K", ctypes.c_double), ("Q", ctypes.c_double),
+                ("damp", ctypes.c_double), ("n_subunits", ctypes.c_int), ("q_chain", ctypes.c_int),
+                ("brownian", ctypes.c_int)]
+
+
+class SfParams(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_double), ("ecut", ctypes.c_double), ("lb", ctypes.c_double),
+                ("kappa", ctypes.c_double), ("ecut_el", ctypes.c_double), ("ks", ctypes.c_double),
+                ("kb", ctypes.c_double), ("elj_att", ctypes.c_double), ("T", ctypes.c_double),
+                ("dt", ctypes.c_double), ("damp", ctypes.c_double), ("Q", ctypes.c_double),
+                ("q_chain", ctypes.c_int), ("brownian", ctypes.c_int), ("skin", ctypes.c_double),
+                ("pair_fp32", ctypes.c_int), ("bd_seed", ctypes.c_uint32)]
+
+
+class SfStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("rebuilds", ctypes.c_int64), ("lj_entries", ctypes.c_int64),
+                ("q_entries", ctypes.c_int64), ("lj_capacity", ctypes.c_int32), ("q_capacity", ctypes.c_int32),
+                ("n_beads", ctypes.c_int32), ("n_subunits", ctypes.c_int32), ("n_charged", ctypes.c_int32),
+                ("n_replicas", ctypes.c_int32), ("k_integrate_ms", ctypes.c_double),
+                ("k_force_ms", ctypes.c_double), ("k_reduce_ms", ctypes.c_double),
+                ("k_timed_steps", ctypes.c_double)]
+
+
+# every entry point of include/souffle_gpu.h: name -> (restype, argtypes)
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+SIGNATURES = {
+    "sf_template_load": (_P, [ctypes.c_char_p]),
+    "sf_template_free": (None, [_P]),
+    "sf_template_info": (ctypes.c_int, [_P, _I, _I, _I, _I, _I, _I, _D, _D]),
+    "sf_template_hinges": (ctypes.c_int, [_P, _I, ctypes.c_int]),
+    "sf_params_derive": (ctypes.c_int, [_P, ctypes.POINTER(SfCli), ctypes.POINTER(SfParams)]),
+    "sf_make_state": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _D, _D]),
+    "sf_create": (_P, [_P, ctypes.c_int, _I, ctypes.POINTER(SfParams)]),
+    "sf_destroy": (None, [_P]),
+    "sf_set_state": (ctypes.c_int, [_P, _D, _D]),
+    "sf_get_state": (ctypes.c_int, [_P, _D, _D]),
+    "sf_get_thermostat": (ctypes.c_int, [_P, ctypes.c_int, _D, _D]),
+    "sf_forces": (ctypes.c_int, [_P, _D, _D, _D, _D, _D]),
+    "sf_energies": (ctypes.c_int, [_P, _D, _D]),
+    "sf_step": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sf_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sf_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(SfStats)]),
+    "sf_bd_uniforms": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _D]),
+    "sf_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (raises OSError if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise OSError(f"{path} is not built: run __graft_entry__.build() or make -C capsid-souffle_amd")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class SfError(RuntimeError):
+    pass
+
+
+def _err(rc, what):
+    if rc < 0:
+        raise SfError(f"{what} failed ({rc}): {_lib.sf_last_error().decode()}")
+    return rc
+
+
+def _dp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_D)
+
+
+class Template:
+    def __init__(self, path):
+        lib = load()
+        self.path = path
+        self.ptr = lib.sf_template_load(path.encode())
+        if not self.ptr:
+            raise SfError(lib.sf_last_error().decode())
+        ints = [ctypes.c_int() for _ in range(6)]
+        dbl = [ctypes.c_double() for _ in range(2)]
+        _err(lib.sf_template_info(self.ptr, *[ctypes.byref(x) for x in ints], *[ctypes.byref(x) for x in dbl]),
+             "sf_template_info")
+        (self.n_beads, self.n_edges, self.n_faces, self.n_hinges, self.n_charged,
+         self.centre) = [x.value for x in ints]
+        self.si_sigma, self.si_mass = [x.value for x in dbl]
+
+    def hinges(self):
+        out = np.zeros((self.n_hinges, 4), dtype=np.int32)
+        load().sf_template_hinges(self.ptr, out.ctypes.data_as(_I), self.n_hinges)
+        return out
+
+    def params(self, **cli):
+        """sf_params from the reference's flags (C_uM, c_mM, ks, kb, elj_att, ecut_c, dt, K, Q,
+        damp, n_subunits, q_chain, brownian); defaults are the reference's defaults."""
+        c = SfCli(C_uM=200.0, c_mM=500.0, ks=50.0, kb=20.0, elj_att=2.0, ecut_c=12.0, dt=0.004,
+                  K=298.0, Q=10.0, damp=100.0, n_subunits=64, q_chain=5, brownian=0)
+        for k, v in cli.items():
+            setattr(c, k, v)
+        p = SfParams()
+        _err(load().sf_params_derive(self.ptr, ctypes.byref(c), ctypes.byref(p)), "sf_params_derive")
+        return p
+
+    def make_state(self, n_subunits, box, T, seed):
+        n = n_subunits * self.n_beads
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        _err(load().sf_make_state(self.ptr, n_subunits, box, T, seed, _dp(pos), _dp(vel)), "sf_make_state")
+        return pos, vel
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.sf_template_free(self.ptr)
+            self.ptr = None
+
+
+class System:
+    """A device-resident system (or batch of replicas) on the current GPU."""
+
+    def __init__(self, template, n_subunits, params):
+        lib = load()
+        if isinstance(n_subunits, int):
+            n_subunits = [n_subunits]
+        if isinstance(params, SfParams):
+            params = [params] * len(n_subunits)
+        self.template = template
+        self.n_subunits = list(n_subunits)
+        self.R = len(n_subunits)
+        ns = (ctypes.c_int * self.R)(*n_subunits)
+        pa = (SfParams * self.R)(*params)
+        self.ptr = lib.sf_create(template.ptr, self.R, ns, pa)
+        if not self.ptr:
+            raise SfError(lib.sf_last_error().decode())
+        self.n_beads = sum(n_subunits) * template.n_beads
+
+    def set_state(self, pos, vel=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+        assert pos.shape == (self.n_beads, 3)
+        _err(_lib.sf_set_state(self.ptr, _dp(pos), _dp(vel)), "sf_set_state")
+
+    def get_state(self):
+        pos = np.zeros((self.n_beads, 3))
+        vel = np.zeros((self.n_beads, 3))
+        _err(_lib.sf_get_state(self.ptr, _dp(pos), _dp(vel)), "sf_get_state")
+        return pos, vel
+
+    def thermostat(self, replica=0):
+        xi = np.zeros(16)
+        eta = np.zeros(16)
+        q = _err(_lib.sf_get_thermostat(self.ptr, replica, _dp(xi), _dp(eta)), "sf_get_thermostat")
+        return xi[:q], eta[:q]
+
+    def forces(self):
+        """(total, stretch, bend, yukawa, lj), each [N,3]."""
+        out = [np.zeros((self.n_beads, 3)) for _ in range(5)]
+        _err(_lib.sf_forces(self.ptr, *[_dp(a) for a in out]), "sf_forces")
+        return out
+
+    def energies(self, per_bead=False):
+        tot = np.zeros((self.R, 7))
+        pb = np.zeros((self.n_beads, 5)) if per_bead else None
+        _err(_lib.sf_energies(self.ptr, _dp(tot), _dp(pb)), "sf_energies")
+        return (tot, pb) if per_bead else tot
+
+    def step(self, n):
+        _err(_lib.sf_step(self.ptr, int(n)), "sf_step")
+
+    def set_timing(self, on):
+        _err(_lib.sf_set_timing(self.ptr, 1 if on else 0), "sf_set_timing")
+
+    def stats(self):
+        s = SfStats()
+        _err(_lib.sf_get_stats(self.ptr, ctypes.byref(s)), "sf_get_stats")
+        return {k: getattr(s, k) for k, _ in SfStats._fields_}
+
+    def bd_uniforms(self, replica, step):
+        nb = self.n_subunits[replica] * self.template.n_beads
+        out = np.zeros((nb, 3))
+        _err(_lib.sf_bd_uniforms(self.ptr, replica, int(step), _dp(out)), "sf_bd_uniforms")
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            _lib.sf_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()

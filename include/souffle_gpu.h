@@ -1,0 +1,168 @@
+/*
+ * souffle_gpu.h — C ABI of the MI355X (gfx950) force-and-integrate engine for SOUFFLE
+ * capsomere models.
+ *
+ * The library replaces the reference's per-step hot path:
+ *   forceCalculation(...)                   src/forces.h:13-15, src/forces.cpp:13-171
+ *   update_LJ_ES_energies_simplified(...)   src/energies.h:10,  src/energies.cpp:24-93
+ *   particle_kinetic_energy(...)            src/energies.h:12,  src/energies.cpp:13-22
+ *   the body of the step loop               src/md.cpp:387-514 (Nose-Hoover MD and "Brownian")
+ * behind plain-pointer entry points. Everything runs on the GPU; the caller owns host arrays,
+ * the library owns device memory. A handle holds one system or a batch of independent replicas
+ * (same template, own box/salt/epsilon/temperature each). Handles are not thread-safe.
+ *
+ * Arrays of bead vectors are AoS [N][3] doubles, beads in the reference's order
+ * (subunit-major, template order inside a subunit, src/initialize.cpp:221-223); for a batch,
+ * replica 0's beads come first, then replica 1's, and so on.
+ *
+ * Every int-returning call returns 0 on success and a negative code on failure
+ * (SF_E_*); sf_last_error() gives the message of the last failure on this thread.
+ */
+#ifndef SOUFFLE_GPU_H
+#define SOUFFLE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SF_OK 0
+#define SF_E_ARG (-1)       /* invalid argument or unsupported template */
+#define SF_E_HIP (-2)       /* HIP runtime error (no device, out of memory, launch failure) */
+#define SF_E_IO (-3)        /* template or file could not be read */
+#define SF_E_OVERFLOW (-4)  /* a neighbour list outgrew its capacity and could not be regrown */
+
+typedef struct sf_template sf_template; /* parsed capsomere template + derived topology */
+typedef struct sf_handle sf_handle;     /* device-resident system or batch of replicas */
+
+/* The reference's command-line values that reach the hot path (src/md.cpp:53-90). */
+typedef struct {
+  double C_uM;    /* -C capsomere concentration, micromolar */
+  double c_mM;    /* -c salt concentration, millimolar */
+  double ks;      /* -s stretching constant (kT) */
+  double kb;      /* -b bending constant (kT) */
+  double elj_att; /* -E attractive LJ well depth */
+  double ecut_c;  /* -e Yukawa cutoff in screening lengths; 0 = no cutoff */
+  double dt;      /* -t timestep */
+  double K;       /* -K temperature (Kelvin) */
+  double Q;       /* -Q Nose-Hoover mass */
+  double damp;    /* -d BD damping */
+  int n_subunits; /* -S */
+  int q_chain;    /* -q Nose-Hoover chain length */
+  int brownian;   /* -D: 1 = 'b', 0 = 'm' */
+} sf_cli;
+
+/* Physical and numerical parameters of one system (one replica). */
+typedef struct {
+  double box;      /* cubic box edge, sigma units */
+  double ecut;     /* LJ cutoff */
+  double lb;       /* Bjerrum length */
+  double kappa;    /* inverse screening length */
+  double ecut_el;  /* Yukawa cutoff; 0 = none (all charged pairs) */
+  double ks, kb, elj_att;
+  double T;        /* reduced temperature K/298 */
+  double dt, damp;
+  double Q;        /* mass of the first Nose-Hoover bath */
+  int q_chain;     /* Nose-Hoover chain length (>= 1) */
+  int brownian;    /* 1 = BD integrator, 0 = MD + Nose-Hoover */
+  double skin;     /* Verlet skin of the GPU lists (sigma); the list is rebuilt on the device
+                      when any bead of the replica has moved more than skin/2 */
+  int pair_fp32;   /* 1 = LJ and Yukawa pair arithmetic in FP32 (positions, velocities,
+                      thermostat and energy sums stay FP64) */
+  uint32_t bd_seed;/* seed of the BD noise engine; the reference uses 69521 */
+} sf_params;
+
+typedef struct {
+  int64_t steps;          /* loop iterations done since sf_set_state */
+  int64_t rebuilds;       /* neighbour-list rebuilds since sf_set_state */
+  int64_t lj_entries;     /* LJ/WCA list entries at the last rebuild (all beads) */
+  int64_t q_entries;      /* Yukawa list entries at the last rebuild (charged beads) */
+  int32_t lj_capacity;    /* current per-bead capacities */
+  int32_t q_capacity;
+  int32_t n_beads, n_subunits, n_charged, n_replicas;
+  double k_integrate_ms;  /* mean kernel durations over the last timed sf_step */
+  double k_force_ms;
+  double k_reduce_ms;
+  double k_timed_steps;   /* steps the means were taken over (0 if timing was off) */
+} sf_stats;
+
+/* ---- template and parameters ---------------------------------------------------------- */
+
+/* Reads a template in the reference's infiles format (src/initialize.cpp:46-128) and derives
+ * the per-template topology: bending hinges (src/initialize.cpp:348-404, src/face.cpp:27-34),
+ * per-bead edge and hinge incidence, charged beads, centre bead. NULL on failure. */
+sf_template* sf_template_load(const char* path);
+void sf_template_free(sf_template* t);
+
+/* Template sizes: any pointer may be NULL. */
+int sf_template_info(const sf_template* t, int* n_beads, int* n_edges, int* n_faces,
+                     int* n_hinges, int* n_charged, int* centre_bead, double* si_sigma,
+                     double* si_mass);
+
+/* Template-local hinge table [n_hinges][4] = (B0, B1, across F0, across F1). */
+int sf_template_hinges(const sf_template* t, int* out, int cap);
+
+/* Derives box, cutoffs, Bjerrum length and kappa from the reference's flags exactly as
+ * src/md.cpp:135-173 does; skin = 0.3, pair_fp32 = 0, bd_seed = 69521. */
+int sf_params_derive(const sf_template* t, const sf_cli* cli, sf_params* out);
+
+/* Seeded initial state: non-overlapping subunits (bead gaps >= 1 sigma) at uniform random
+ * centres and uniform random orientations in [-box/2, box/2)^3, Maxwell-Boltzmann velocities
+ * at T, centre-of-mass velocity removed and kinetic energy rescaled to exactly 1.5 N T.
+ * Host-only. pos, vel: [n_subunits * n_beads][3]. */
+int sf_make_state(const sf_template* t, int n_subunits, double box, double T, uint64_t seed,
+                  double* pos, double* vel);
+
+/* ---- device system -------------------------------------------------------------------- */
+
+/* One handle for n_replicas independent systems; n_subunits[r] and params[r] per replica.
+ * Allocates device memory on the current HIP device. NULL on failure. */
+sf_handle* sf_create(const sf_template* t, int n_replicas, const int* n_subunits,
+                     const sf_params* params);
+void sf_destroy(sf_handle* h);
+
+/* Uploads positions and velocities (all replicas) and (re)starts every replica as the
+ * reference's run start does: thermostat xi = eta = 0, BD engine reseeded, step counter 0,
+ * neighbour lists built, forces computed (src/md.cpp:241) and the kinetic energy taken
+ * (src/md.cpp:258). vel may be NULL (zero velocities). */
+int sf_set_state(sf_handle* h, const double* pos, const double* vel);
+
+/* Downloads positions and velocities (either may be NULL). */
+int sf_get_state(sf_handle* h, double* pos, double* vel);
+
+/* Nose-Hoover chain variables of one replica (arrays of q_chain); returns q_chain. */
+int sf_get_thermostat(sf_handle* h, int replica, double* xi, double* eta);
+
+/* Forces at the current positions with a fresh list build: the forceCalculation equivalent.
+ * Outputs [N][3], any may be NULL: total = stretch + bend + (Yukawa + LJ), and the four terms.
+ * Does not change the integrator state. */
+int sf_forces(sf_handle* h, double* f_total, double* f_stretch, double* f_bend,
+              double* f_yukawa, double* f_lj);
+
+/* Energies at the current state (src/md.cpp:553-605): totals[n_replicas][7] =
+ * {kinetic, stretching, bending, LJ, Yukawa, thermostat potential, thermostat kinetic};
+ * per_bead (nullable) [N][5] = {se, be, ne, ce, ke} as the reference's BEAD fields. */
+int sf_energies(sf_handle* h, double* totals, double* per_bead);
+
+/* Runs n_steps iterations of the step loop body (src/md.cpp:400-514) for every replica. */
+int sf_step(sf_handle* h, int n_steps);
+
+/* Per-kernel timing with HIP events on the library's stream during sf_step (off by default). */
+int sf_set_timing(sf_handle* h, int enable);
+
+int sf_get_stats(sf_handle* h, sf_stats* out);
+
+/* The BD noise draws of one replica for loop iteration `step` (0 = first iteration after
+ * sf_set_state): [n_beads_of_replica][3] uniform(-0.5, 0.5) values, the stream of the
+ * reference's std::default_random_engine(seed) + uniform_real_distribution(-0.5, 0.5)
+ * (src/md.cpp:382-385, 464-466), computed on the GPU by skip-ahead. */
+int sf_bd_uniforms(sf_handle* h, int replica, int64_t step, double* out);
+
+const char* sf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOUFFLE_GPU_H */

@@ -1,0 +1,183 @@
+"""GPU path (through the C ABI) against the CPU oracle on identical seeded states.
+
+Tolerances:
+  forces and energies: 1e-10 norm-wise relative (the reference's own arithmetic is x87 long
+  double; the FP64 GPU path and the FP64 oracle differ by summation order only);
+  trajectories: positions/velocities after 100-200 steps within 1e-8 (chaotic growth of
+  round-off stays far below this over one reduced time unit);
+  BD noise: bit-exact against libstdc++'s std::default_random_engine stream.
+"""
+import numpy as np
+import pytest
+
+import souffle_gpu as sg
+from oracle_lib import TEMPLATE, TEMPLATE_BIN, Oracle, bd_uniforms, rel_norm
+from souffle_states import make_state, read_template_xyz
+
+pytestmark = pytest.mark.gpu
+
+CFG0 = dict(C_uM=800.0, c_mM=100.0, elj_att=2.1, dt=0.005, K=298.0)
+
+
+def _system(ns, template=TEMPLATE, skin=0.3, fp32=False, brownian=False, q_chain=5, **over):
+    t = sg.Template(template)
+    cli = dict(CFG0, n_subunits=ns, brownian=1 if brownian else 0, q_chain=q_chain)
+    cli.update(over)
+    p = t.params(**cli)
+    p.skin = skin
+    p.pair_fp32 = 1 if fp32 else 0
+    return t, p, sg.System(t, ns, p)
+
+
+def _state(template, ns, box, seed, spacing, perturb=0.05):
+    xyz = read_template_xyz(template)[0]
+    return make_state(xyz, ns, box, seed, spacing=spacing, perturb=perturb)
+
+
+def _oracle(ns, template=TEMPLATE, brownian=False, q_chain=5, long_double=False, **over):
+    kw = dict(CFG0)
+    kw.update(over)
+    return Oracle(ns, template=template, brownian=brownian, q_chain=q_chain, long_double=long_double, **kw)
+
+
+@pytest.mark.parametrize("template,seed,spacing", [(TEMPLATE, 1, 8.6), (TEMPLATE, 2, 8.3), (TEMPLATE_BIN, 3, 8.8)])
+def test_forces_match_oracle(template, seed, spacing):
+    t, p, sysg = _system(20, template)
+    pos, vel = _state(template, 20, p.box, seed, spacing)
+    o = _oracle(20, template)
+    o.set_state(pos, vel)
+    ref = o.forces()
+    sysg.set_state(pos, vel)
+    got = sysg.forces()
+    names = ["total", "stretch", "bend", "yukawa", "lj"]
+    for name, g, r in zip(names, got, ref):
+        if np.linalg.norm(r) == 0:
+            assert np.linalg.norm(g) == 0, name
+            continue
+        assert rel_norm(g, r) < 1e-10, (name, rel_norm(g, r))
+    assert np.linalg.norm(ref[4]) > 0, "state has no LJ contacts"
+    if template == TEMPLATE:
+        assert np.linalg.norm(ref[3]) > 0
+
+
+def test_energies_match_oracle():
+    t, p, sysg = _system(20)
+    pos, vel = _state(TEMPLATE, 20, p.box, 4, 8.6)
+    o = _oracle(20)
+    o.set_state(pos, vel)
+    ref, ref_pb = o.energies(per_bead=True)
+    sysg.set_state(pos, vel)
+    got, got_pb = sysg.energies(per_bead=True)
+    got = got[0]
+    for k, name in enumerate(["ke", "se", "be", "ne", "ce"]):
+        assert abs(got[k] - ref[k]) <= 1e-10 * max(abs(ref[k]), 1e-300), (name, got[k], ref[k])
+    # per bead: se, be, ne, ce, ke in the reference's BEAD fields
+    for k in range(5):
+        assert rel_norm(got_pb[:, k], ref_pb[:, k]) < 1e-10, k
+
+
+@pytest.mark.parametrize("q_chain", [5, 1])
+def test_md_trajectory_matches_oracle(q_chain):
+    t, p, sysg = _system(20, q_chain=q_chain)
+    pos, vel = _state(TEMPLATE, 20, p.box, 5, 8.6)
+    o = _oracle(20, q_chain=q_chain, L=3.0, B=5)
+    o.set_state(pos, vel)
+    sysg.set_state(pos, vel)
+    n = 200
+    o.run(n)
+    sysg.step(n)
+    op, ov, oxi, oeta = o.get_state()
+    gp, gv = sysg.get_state()
+    assert rel_norm(gp, op) < 1e-8
+    assert rel_norm(gv, ov) < 1e-8
+    gxi, geta = sysg.thermostat()
+    np.testing.assert_allclose(gxi, oxi, rtol=1e-8, atol=1e-12)
+    np.testing.assert_allclose(geta, oeta, rtol=1e-8, atol=1e-12)
+    ge = sysg.energies()[0]
+    oe = o.energies()
+    np.testing.assert_allclose(ge, oe, rtol=1e-8, atol=1e-9)
+    assert sysg.stats()["steps"] == n
+
+
+def test_bd_noise_stream_bit_exact():
+    t, p, sysg = _system(20, brownian=True)
+    nb = 20 * t.n_beads
+    for step in [0, 1, 7]:
+        got = sysg.bd_uniforms(0, step).reshape(-1)
+        ref = bd_uniforms(step * 3 * nb, 3 * nb)
+        assert np.array_equal(got, ref), step
+
+
+def test_bd_trajectory_matches_oracle():
+    t, p, sysg = _system(20, brownian=True)
+    pos, vel = _state(TEMPLATE, 20, p.box, 6, 8.6)
+    o = _oracle(20, brownian=True, L=3.0, B=5)
+    o.set_state(pos, vel)
+    sysg.set_state(pos, vel)
+    n = 100
+    o.run(n)
+    sysg.step(n)
+    op, ov, _, _ = o.get_state()
+    gp, gv = sysg.get_state()
+    assert rel_norm(gp, op) < 1e-8
+    assert rel_norm(gv, ov) < 1e-8
+
+
+def test_deterministic_rerun():
+    t, p, sysg = _system(20)
+    pos, vel = _state(TEMPLATE, 20, p.box, 8, 8.6)
+    outs = []
+    for _ in range(2):
+        sysg.set_state(pos, vel)
+        sysg.step(150)
+        outs.append(sysg.get_state())
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_fp32_pair_form_error():
+    t, p64, s64 = _system(20)
+    _, p32, s32 = _system(20, fp32=True)
+    pos, vel = _state(TEMPLATE, 20, p64.box, 9, 8.6)
+    s64.set_state(pos, vel)
+    s32.set_state(pos, vel)
+    f64 = s64.forces()[0]
+    f32 = s32.forces()[0]
+    err = rel_norm(f32, f64)
+    assert err < 1e-3, err
+    assert err > 0
+
+
+def test_replicas_match_single_runs():
+    t = sg.Template(TEMPLATE)
+    sizes = [20, 27, 12]
+    params = []
+    for k, ns in enumerate(sizes):
+        p = t.params(C_uM=800.0, c_mM=[100.0, 300.0, 150.0][k], elj_att=[2.1, 1.6, 2.4][k], dt=0.005,
+                     K=[298.0, 285.0, 310.0][k], n_subunits=ns)
+        params.append(p)
+    states = [_state(TEMPLATE, ns, params[k].box, 20 + k, 8.7) for k, ns in enumerate(sizes)]
+    batch = sg.System(t, sizes, params)
+    batch.set_state(np.concatenate([s[0] for s in states]), np.concatenate([s[1] for s in states]))
+    batch.step(120)
+    bp, bv = batch.get_state()
+    off = 0
+    for k, ns in enumerate(sizes):
+        single = sg.System(t, ns, params[k])
+        single.set_state(*states[k])
+        single.step(120)
+        sp, sv = single.get_state()
+        n = ns * t.n_beads
+        assert np.array_equal(bp[off:off + n], sp), k
+        assert np.array_equal(bv[off:off + n], sv), k
+        off += n
+
+
+def test_dense_state_regrows_lists():
+    t, p, sysg = _system(27)
+    pos, vel = _state(TEMPLATE, 27, p.box, 11, 6.0, perturb=0.02)
+    o = _oracle(27)
+    o.set_state(pos, vel)
+    ref = o.forces()[0]
+    sysg.set_state(pos, vel)
+    got = sysg.forces()[0]
+    assert rel_norm(got, ref) < 1e-10
