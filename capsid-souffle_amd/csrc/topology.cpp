@@ -166,7 +166,7 @@ sf_template* sf_template_load(const char* path) {
     in >> name >> a >> b >> eps >> sig;
     t->la1.push_back(a); t->la2.push_back(b);
   }
-  if (!in || t->np <= 0 || t->ne <= 0 || t->nf <= 0 || (int)t->x.size() != t->np ||
+  if (!in || t->np <= 0 || t->ne < 0 || t->nf < 0 || (int)t->x.size() != t->np ||
       (int)t->e1.size() != t->ne || (int)t->f1.size() != t->nf) {
     set_error(std::string("malformed template ") + path);
     delete t;

@@ -113,7 +113,7 @@ bool read_template(const char* path, Tmpl& t) {
     in >> name >> a >> b >> eps >> sig;
     t.la1.push_back(a); t.la2.push_back(b);
   }
-  if (!in || t.np <= 0 || t.ne <= 0 || t.nf <= 0) { g_err = "malformed template"; return false; }
+  if (!in || t.np <= 0 || t.ne < 0 || t.nf < 0) { g_err = "malformed template"; return false; }
   return true;
 }
 

@@ -31,8 +31,9 @@ def random_rotation(rng):
 
 
 def make_state(tmpl_xyz, n_sub, box, seed, spacing=None, jitter=0.5, perturb=0.05, T=1.0,
-               masses=None, wrap=True):
-    """Return (pos, vel) as [N,3] float64 arrays, N = n_sub * n_template_beads."""
+               masses=None, wrap=True, min_gap=0.9):
+    """Return (pos, vel) as [N,3] float64 arrays, N = n_sub * n_template_beads. Each subunit's
+    rotation is redrawn until its beads are at least `min_gap` from all beads placed before."""
     rng = np.random.default_rng(seed)
     np_ = tmpl_xyz.shape[0]
     com = tmpl_xyz.mean(axis=0)
@@ -48,8 +49,18 @@ def make_state(tmpl_xyz, n_sub, box, seed, spacing=None, jitter=0.5, perturb=0.0
                 if k >= n_sub:
                     break
                 c = (np.array([i, j, l]) + 0.5 - nfill / 2.0) * spacing + rng.uniform(-jitter, jitter, 3)
-                R = random_rotation(rng)
-                pos[k * np_:(k + 1) * np_] = body @ R.T + c
+                for attempt in range(2000):
+                    R = random_rotation(rng)
+                    trial = body @ R.T + c
+                    if k == 0:
+                        break
+                    d = trial[:, None, :] - pos[None, :k * np_, :]
+                    d = np.where(d > box / 2, d - box, np.where(d < -box / 2, d + box, d))
+                    if np.min(np.einsum("ijk,ijk->ij", d, d)) >= min_gap * min_gap:
+                        break
+                else:
+                    raise RuntimeError("could not place subunits without overlap; increase spacing")
+                pos[k * np_:(k + 1) * np_] = trial
                 k += 1
     pos += rng.normal(scale=perturb, size=pos.shape)
     if wrap:
