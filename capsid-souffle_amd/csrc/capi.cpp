@@ -43,7 +43,7 @@ struct sf_handle {
   int mode = MODE_MD;
   int fp32 = 0;
   bool timing = false;
-  double t_int = 0, t_force = 0, t_red = 0, t_steps = 0;
+  double t_int = 0, t_rb = 0, t_force = 0, t_red = 0, t_steps = 0;
   // snapshot for replaying a chunk
   double4* s_pos = nullptr; double* s_v = nullptr; double* s_f = nullptr; double* s_g = nullptr;
   RepState* s_rs = nullptr;
@@ -65,6 +65,8 @@ struct sf_handle {
     if (it != allocs.end()) { allocs.erase(it); (void)hipFree(p); }
   }
   ~sf_handle() {
+    drop_graph();
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
   }
@@ -83,10 +85,16 @@ struct sf_handle {
       double rlj = kWcaFactor * smax;
       if (any_att) rlj = std::max(rlj, p.ecut);
       rlj += p.skin;
+      P.rl_lj = rlj;
       P.sub_rc_lj2 = (rlj + 2 * rb) * (rlj + 2 * rb);
       const double rql = p.ecut_el + p.skin;
       P.sub_rc_q2 = (rql + 2 * rq) * (rql + 2 * rq);
+      P.r_bound = rb;
       P.r_bound2 = rb * rb;
+      P.r_bound_q2 = rq * rq;
+      // the LDS build grid spans +-(r_bound + rl_lj) around the centre bead and must stay
+      // inside half a box; bead ids must fit the 2^23 float index of the grid records
+      P.grid_ok = (rb + rlj + 2e-3 < 0.5 * p.box && (long long)Nb < (1ll << 23)) ? 1 : 0;
     }
   }
 
@@ -179,8 +187,9 @@ struct sf_handle {
   int start_sequence() {
     for (int attempt = 0; attempt < 16; attempt++) {
       if (int rc = launch_centres(d, st)) return rc;
-      if (int rc = launch_force(d, MODE_FORCES, 1, 1, fp32, st)) return rc;
-      if (int rc = launch_finalize(d, mode, 1, st)) return rc;
+      if (int rc = launch_rebuild(d, 1, st)) return rc;
+      if (int rc = launch_force(d, MODE_FORCES, 1, fp32, st)) return rc;
+      if (int rc = launch_finalize(d, 1, st)) return rc;
       int rc = check_and_fix();
       if (rc < 0) return rc;
       if (rc == 0) return SF_OK;
@@ -193,18 +202,71 @@ struct sf_handle {
     return SF_E_OVERFLOW;
   }
 
-  int run_chunk(int n, std::vector<hipEvent_t>* ev) {
+  // n steps of the loop body: integrate, (conditional) list build, forces, reductions. With
+  // timing on, every kTimeEvery-th step's kernels are bracketed by events (5 per sample).
+  int run_chunk(int n, bool tm_on) {
     for (int k = 0; k < n; k++) {
-      const bool tm = ev && (k % kTimeEvery == 0);
-      hipEvent_t* e = tm ? &(*ev)[4 * (k / kTimeEvery)] : nullptr;
+      const bool tm = tm_on && (k % kTimeEvery == 0);
+      hipEvent_t* e = tm ? &events[5 * (k / kTimeEvery)] : nullptr;
       if (tm) HIPCHK(hipEventRecord(e[0], st));
       if (int rc = launch_integrate(d, mode, st)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[1], st));
-      if (int rc = launch_force(d, mode, 0, 0, fp32, st)) return rc;
+      if (int rc = launch_rebuild(d, 0, st)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[2], st));
-      if (int rc = launch_finalize(d, mode, 0, st)) return rc;
+      if (int rc = launch_force(d, mode, 0, fp32, st)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[3], st));
+      if (int rc = launch_finalize(d, 0, st)) return rc;
+      if (tm) HIPCHK(hipEventRecord(e[4], st));
     }
+    return SF_OK;
+  }
+
+  // The chunk is captured once into a HIP graph and replayed; the capture is redone when the
+  // chunk length, the timing mode or any list buffer (reallocated on growth) changes.
+  hipGraphExec_t gexec = nullptr;
+  int g_n = -1;
+  bool g_tm = false;
+  const void* g_key[2] = {nullptr, nullptr};
+  int g_cap[2] = {0, 0};
+  std::vector<hipEvent_t> events;
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  }
+  int ensure_events() {
+    const size_t need = 5 * ((kChunk + kTimeEvery - 1) / kTimeEvery);
+    while (events.size() < need) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      events.push_back(e);
+    }
+    return SF_OK;
+  }
+  int launch_chunk(int n) {
+    if (timing) {
+      // HIP does not take timing events into a captured graph: timed chunks launch directly
+      if (int rc = ensure_events()) return rc;
+      return run_chunk(n, true);
+    }
+    // the captured kernel arguments hold the list buffers and their capacities; a regrown
+    // buffer may come back at the same address, so the capacities are part of the key
+    if (!gexec || g_n != n || g_tm != timing || g_key[0] != d.lj_list || g_key[1] != d.q_list ||
+        g_cap[0] != d.lj_cap || g_cap[1] != d.q_cap) {
+      drop_graph();
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = run_chunk(n, timing);
+      hipError_t ec = hipStreamEndCapture(st, &g);
+      if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+      if (ec != hipSuccess) { set_error(std::string("graph capture failed: ") + hipGetErrorString(ec)); return SF_E_HIP; }
+      hipError_t ei = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess) { gexec = nullptr; set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return SF_E_HIP; }
+      g_n = n; g_tm = timing; g_key[0] = d.lj_list; g_key[1] = d.q_list;
+      g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap;
+    }
+    HIPCHK(hipGraphLaunch(gexec, st));
     return SF_OK;
   }
 };
@@ -268,10 +330,32 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
     *dst = p;
     return SF_OK;
   };
+  // padded per-bead rows of incident edges and hinge incidences
+  std::vector<int> e_cnt(np), h_cnt(np);
+  std::vector<unsigned char> e_oth((size_t)np * kMaxDeg, 0);
+  std::vector<unsigned short> h_sl((size_t)np * kMaxHinc, 0);
+  std::vector<double> e_l0((size_t)np * kMaxDeg, 0.0);
+  if (4 * t->nh() > 65535) { set_error("template has too many bending hinges"); delete h; return nullptr; }
+  for (int l = 0; l < np; l++) {
+    e_cnt[l] = t->e_off[l + 1] - t->e_off[l];
+    h_cnt[l] = t->h_off[l + 1] - t->h_off[l];
+    if (e_cnt[l] > kMaxDeg || h_cnt[l] > kMaxHinc) {
+      set_error("template bead with more than 8 edges or 16 hinge incidences");
+      delete h;
+      return nullptr;
+    }
+    for (int k = 0; k < e_cnt[l]; k++) {
+      e_oth[(size_t)l * kMaxDeg + k] = t->e_other[t->e_off[l] + k];
+      e_l0[(size_t)l * kMaxDeg + k] = t->e_len0[t->e_off[l] + k];
+    }
+    for (int k = 0; k < h_cnt[l]; k++) h_sl[(size_t)l * kMaxHinc + k] = t->h_slot[t->h_off[l] + k];
+  }
+  std::vector<int> hinges = t->hinges;
+  if (hinges.empty()) hinges.assign(4, 0);
   if (up((double**)&td.mass, t->mass) || up((double**)&td.sigma, t->sigma) || up((int**)&td.type, t->type) ||
-      up((int**)&td.qslot, t->qslot) || up((int**)&td.qlocal, t->qlocal) || up((int**)&td.e_off, t->e_off) ||
-      up((int**)&td.e_other, t->e_other) || up((double**)&td.e_len0, t->e_len0) ||
-      up((int**)&td.hinge, t->hinges) || up((int**)&td.h_off, t->h_off) || up((int**)&td.h_slot, t->h_slot) ||
+      up((int**)&td.qslot, t->qslot) || up((int**)&td.qlocal, t->qlocal) || up((int**)&td.e_cnt, e_cnt) ||
+      up((unsigned char**)&td.e_other, e_oth) || up((double**)&td.e_len0, e_l0) || up((int**)&td.h_cnt, h_cnt) ||
+      up((unsigned short**)&td.h_slot, h_sl) || up((int**)&td.hinge, hinges) ||
       up((unsigned long long**)&td.att_mask, t->att_mask))
     return fail(nullptr);
   // per-replica tables: qq [nq*nq], yk [nq*nq], bd_amp [np]
@@ -427,7 +511,8 @@ int sf_forces(sf_handle* h, double* f, double* fs, double* fb, double* fe, doubl
   for (int attempt = 0; attempt < 16; attempt++) {
     d.lj_list = h->d.lj_list; d.q_list = h->d.q_list; d.lj_cap = h->d.lj_cap; d.q_cap = h->d.q_cap;
     if (int rc = launch_centres(d, h->st)) return rc;
-    if (int rc = launch_force(d, MODE_FORCES, 0, 1, h->fp32, h->st)) return rc;
+    if (int rc = launch_rebuild(d, 1, h->st)) return rc;
+    if (int rc = launch_force(d, MODE_FORCES, 0, h->fp32, h->st)) return rc;
     int rc = h->check_and_fix();
     if (rc < 0) return rc;
     if (rc == 0) break;
@@ -471,18 +556,13 @@ int sf_energies(sf_handle* h, double* totals, double* per_bead) {
 
 int sf_step(sf_handle* h, int n_steps) {
   if (!h || n_steps < 0) { set_error("bad arguments to sf_step"); return SF_E_ARG; }
-  std::vector<hipEvent_t> ev;
-  if (h->timing) {
-    ev.resize(4 * ((kChunk + kTimeEvery - 1) / kTimeEvery));
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-  }
-  double ti = 0, tf = 0, tr = 0, nt = 0;
+  double ti = 0, trb = 0, tf = 0, tr = 0, nt = 0;
   int done = 0, rc = SF_OK;
   int retries = 0;
   while (done < n_steps) {
     const int n = std::min(kChunk, n_steps - done);
     if ((rc = h->snapshot(true))) break;
-    if ((rc = h->run_chunk(n, h->timing ? &ev : nullptr))) break;
+    if ((rc = h->launch_chunk(n))) break;
     rc = h->check_and_fix();
     if (rc < 0) break;
     if (rc == 1) {
@@ -494,23 +574,23 @@ int sf_step(sf_handle* h, int n_steps) {
     }
     if (h->timing) {
       for (int k = 0; k < n; k += kTimeEvery) {
-        const int i = k / kTimeEvery;
-        float a = 0, b = 0, c = 0;
-        (void)hipEventElapsedTime(&a, ev[4 * i], ev[4 * i + 1]);
-        (void)hipEventElapsedTime(&b, ev[4 * i + 1], ev[4 * i + 2]);
-        (void)hipEventElapsedTime(&c, ev[4 * i + 2], ev[4 * i + 3]);
-        ti += a; tf += b; tr += c;
+        const hipEvent_t* e = &h->events[5 * (k / kTimeEvery)];
+        float a = 0, b = 0, c = 0, x = 0;
+        (void)hipEventElapsedTime(&a, e[0], e[1]);
+        (void)hipEventElapsedTime(&b, e[1], e[2]);
+        (void)hipEventElapsedTime(&c, e[2], e[3]);
+        (void)hipEventElapsedTime(&x, e[3], e[4]);
+        ti += a; trb += b; tf += c; tr += x;
         nt += 1;
       }
     }
     done += n;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
   if (rc) return rc;
   if ((rc = launch_apply_com(h->d, h->st))) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
   if (h->timing && nt > 0) {
-    h->t_int = ti / nt; h->t_force = tf / nt; h->t_red = tr / nt; h->t_steps = nt;
+    h->t_int = ti / nt; h->t_rb = trb / nt; h->t_force = tf / nt; h->t_red = tr / nt; h->t_steps = nt;
   }
   return SF_OK;
 }
@@ -530,7 +610,8 @@ int sf_get_stats(sf_handle* h, sf_stats* o) {
   for (auto& x : s) { o->rebuilds += x.rebuilds; o->lj_entries += x.lj_entries; o->q_entries += x.q_entries; }
   o->lj_capacity = h->d.lj_cap; o->q_capacity = h->d.q_cap;
   o->n_beads = h->Nb; o->n_subunits = h->Ns; o->n_charged = h->Nq; o->n_replicas = h->R;
-  o->k_integrate_ms = h->t_int; o->k_force_ms = h->t_force; o->k_reduce_ms = h->t_red; o->k_timed_steps = h->t_steps;
+  o->k_integrate_ms = h->t_int; o->k_rebuild_ms = h->t_rb; o->k_force_ms = h->t_force; o->k_reduce_ms = h->t_red;
+  o->k_timed_steps = h->t_steps;
   return SF_OK;
 }
 

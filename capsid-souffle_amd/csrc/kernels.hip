@@ -1,14 +1,18 @@
-// HIP kernels for gfx950 (MI355X): one SOUFFLE step is
-//   k_integrate  one thread per bead: pending COM removal, first half kick (NH) or drag +
-//                half kick (BD), drift, periodic wrap, list-displacement check
-//   k_force      one 128-thread workgroup per subunit: (list rebuild when flagged) +
-//                stretch + bending (hinge terms staged in LDS, gathered per bead through the
-//                template's incidence CSR) + LJ/WCA and Yukawa gathers over per-bead lists +
-//                second half kick (NH) or noise + drag + half kick (BD) + per-subunit sums
+// HIP kernels for gfx950 (MI355X). One SOUFFLE step is four launches:
+//   k_integrate  one thread per bead: pending centre-of-mass removal, first half kick (NH) or
+//                drag + half kick (BD), drift, periodic wrap, list-displacement check
+//   k_rebuild    one 128-thread workgroup per subunit, returns at once unless a bead of its
+//                replica moved more than skin/2: Verlet lists of the subunit's beads from a
+//                local LDS cell grid of the nearby subunits' beads
+//   k_force      one 128-thread workgroup per subunit: stretch + bending (hinge terms staged in
+//                LDS, gathered per bead through padded incidence rows) + LJ/WCA and Yukawa
+//                gathers over the lists + second half kick (NH) or noise + drag + half kick (BD)
+//                + per-subunit sums
 //   k_finalize   one workgroup per replica: fixed-order reductions, Nose-Hoover chain,
 //                centre-of-mass velocity, BD engine advance
-// No atomics touch forces or sums: each bead's force is gathered by one thread in a fixed
-// order and every reduction has a fixed shape, so runs are bit-reproducible.
+// No atomics touch forces or sums: each bead's force is gathered by one thread, its lists are
+// in ascending bead id, and every reduction has a fixed shape, so runs are bit-reproducible and
+// do not depend on when lists were rebuilt.
 #include <hip/hip_runtime.h>
 
 #include "sf_internal.hpp"
@@ -17,6 +21,11 @@ namespace sf {
 
 __device__ __forceinline__ double min_img(double v, double box, double hbox) {
   // single conditional shift per axis, as src/forces.cpp:85-91 and src/functions.h:42-56
+  if (v > hbox) v -= box;
+  else if (v < -hbox) v += box;
+  return v;
+}
+__device__ __forceinline__ float min_imgf(float v, float box, float hbox) {
   if (v > hbox) v -= box;
   else if (v < -hbox) v += box;
   return v;
@@ -109,8 +118,11 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
   const int r = d.sub_rep[s];
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
+  const double box = P.box, hbox = P.hbox, dt = P.dt, half_skin2 = P.half_skin2;
   const double m = d.topo.mass[l];
-  double vx = d.vx[b] - S.mean_v[0], vy = d.vy[b] - S.mean_v[1], vz = d.vz[b] - S.mean_v[2];
+  const double mvx = S.mean_v[0], mvy = S.mean_v[1], mvz = S.mean_v[2];
+  const double4 q = d.xb[b];
+  double vx = d.vx[b] - mvx, vy = d.vy[b] - mvy, vz = d.vz[b] - mvz;
   const double fx = d.fx[b], fy = d.fy[b], fz = d.fz[b];
   if (MODE == MODE_MD) {
     const double e = S.expfac, c = S.kick_c / m;
@@ -118,74 +130,92 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
   } else {
     const double cd = -1.0 * m / P.damp;
     d.gx[b] = vx * cd; d.gy[b] = vy * cd; d.gz[b] = vz * cd;
-    const double c = 0.5 * P.dt / m;
+    const double c = 0.5 * dt / m;
     vx = vx + fx * c; vy = vy + fy * c; vz = vz + fz * c;
   }
-  const double box = P.box, hbox = P.hbox;
-  p.x = p.x + vx * P.dt; p.y = p.y + vy * P.dt; p.z = p.z + vz * P.dt;
+  p.x = p.x + vx * dt; p.y = p.y + vy * dt; p.z = p.z + vz * dt;
   if (p.x > hbox) p.x = p.x - box; else if (p.x < -hbox) p.x = p.x + box;
   if (p.y > hbox) p.y = p.y - box; else if (p.y < -hbox) p.y = p.y + box;
   if (p.z > hbox) p.z = p.z - box; else if (p.z < -hbox) p.z = p.z + box;
   d.pos[b] = p;
   d.vx[b] = vx; d.vy[b] = vy; d.vz[b] = vz;
-  const double4 q = d.xb[b];
   const double ddx = min_img(p.x - q.x, box, hbox), ddy = min_img(p.y - q.y, box, hbox),
                ddz = min_img(p.z - q.z, box, hbox);
-  if (ddx * ddx + ddy * ddy + ddz * ddz > P.half_skin2) S.rebuild = 1;  // same value from any thread
+  if (ddx * ddx + ddy * ddy + ddz * ddz > half_skin2) S.rebuild = 1;  // same value from any thread
   if (l == d.topo.centre) d.centres[s] = p;
 }
 
 // ------------------------------------------------------------------------------------------
-// Neighbour-list build for the beads of subunit s. Candidate subunits are those whose centre
-// beads lie within the bound (pair cutoff + skin + twice the centre-to-bead radius); they are
-// taken in ascending id, and each one's beads are staged in LDS and tested against every bead
-// of s, so each list comes out in ascending global bead id, the order in which the
-// reference's update_pairlist appends (src/functions.cpp:93-110).
-// LJ list: beads of other subunits within (cutoff of the pair + skin), the cutoff being ecut
-// for an attractive (type_i, type_j) row of lj_a and 1.12246205 (s_i + s_j)/2 otherwise
-// (src/forces.cpp:112-138). Yukawa list: charged beads j != i of any subunit (src/forces.cpp:79)
-// within ecut_el + skin.
-__device__ void build_lists(const Dev& d, const RepParams& P, RepState& S, int s, int tid,
-                            const double4* own, double4* stage, int* cand, unsigned char* cflag,
-                            int* wcnt, int* ncand_sh, int& cnt_lj, int& cnt_q) {
-  const int np = d.topo.np, nq = d.topo.nq;
+// Neighbour lists.
+// LJ list of bead i: beads j of other subunits within (cutoff of the pair + skin), the cutoff
+// being ecut for an attractive (type_i, type_j) row of lj_a and 1.12246205 (s_i + s_j)/2
+// otherwise (src/forces.cpp:112-138). Yukawa list of a charged bead: charged beads j != i of
+// any subunit (src/forces.cpp:79) within ecut_el + skin. Both lists hold ascending global
+// bead ids, the order in which the reference's update_pairlist appends
+// (src/functions.cpp:93-110). Candidate subunits are those whose centre beads lie within
+// the pair reach plus twice the centre-to-bead bound.
+
+// Candidate subunits of s in ascending id: each thread scans a contiguous block of subunit
+// ids so that one workgroup prefix sum places all candidates in order.
+__device__ int find_candidates(const Dev& d, const RepParams& P, RepState& S, int s, int tid, double4 cs,
+                               int* cand, unsigned char* cflag, int* scan) {
   const double box = P.box, hbox = P.hbox;
-  const double4 cs = d.centres[s];
-  if (tid == 0) *ncand_sh = 0;
-  __syncthreads();
-  for (int base = 0; base < P.ns; base += kBlock) {
-    const int t = P.s0 + base + tid;
-    int flag = 0;
-    if (base + tid < P.ns) {
+  const int nq = d.topo.nq;
+  int total = 0;
+  for (int base = 0; base < P.ns; base += 64 * kBlock) {
+    const int span = min(P.ns - base, 64 * kBlock);
+    const int per = (span + kBlock - 1) / kBlock;
+    unsigned long long bl = 0, bq = 0;
+    const int t0 = P.s0 + base + tid * per;
+    for (int m = 0; m < per; m++) {
+      const int t = t0 + m;
+      if (t >= P.s0 + base + span) break;
       const double4 ct = d.centres[t];
       const double dx = min_img(cs.x - ct.x, box, hbox), dy = min_img(cs.y - ct.y, box, hbox),
                    dz = min_img(cs.z - ct.z, box, hbox);
       const double d2 = dx * dx + dy * dy + dz * dz;
-      if (t != s && d2 < P.sub_rc_lj2) flag |= 1;
-      if (nq > 0 && d2 < P.sub_rc_q2) flag |= 2;
+      if (t != s && d2 < P.sub_rc_lj2) bl |= 1ull << m;
+      if (nq > 0 && d2 < P.sub_rc_q2) bq |= 1ull << m;
     }
-    const unsigned long long bal = __ballot(flag != 0);
-    const int lane = tid & 63, wave = tid >> 6;
-    const int prefix = __popcll(bal & ((1ull << lane) - 1ull));
-    if (lane == 0) wcnt[wave] = __popcll(bal);
+    const int cnt = __popcll(bl | bq);
+    scan[tid] = cnt;
     __syncthreads();
-    const int off = *ncand_sh + (wave ? wcnt[0] : 0);
-    if (flag) {
-      const int slot = off + prefix;
-      if (slot < kMaxCand) { cand[slot] = t; cflag[slot] = (unsigned char)flag; }
+    for (int o = 1; o < kBlock; o <<= 1) {  // inclusive Hillis-Steele scan
+      const int v = (tid >= o) ? scan[tid - o] : 0;
+      __syncthreads();
+      scan[tid] += v;
+      __syncthreads();
     }
-    __syncthreads();
-    if (tid == 0) *ncand_sh += wcnt[0] + wcnt[1];
+    int slot = total + scan[tid] - cnt;
+    for (int m = 0; m < per; m++) {
+      const unsigned long long bit = 1ull << m;
+      if ((bl | bq) & bit) {
+        if (slot < kMaxCand) {
+          cand[slot] = t0 + m;
+          cflag[slot] = (unsigned char)(((bl & bit) ? 1 : 0) | ((bq & bit) ? 2 : 0));
+        }
+        slot++;
+      }
+    }
+    total += scan[kBlock - 1];
     __syncthreads();
   }
-  int nc = *ncand_sh;
-  if (nc > kMaxCand) {
-    if (tid == 0) atomicMax(&S.need_cand, nc);
-    nc = kMaxCand;
+  if (total > kMaxCand) {
+    if (tid == 0) atomicMax(&S.need_cand, total);
+    total = kMaxCand;
   }
+  return total;
+}
+
+// Exact FP64 build over the candidates, one staged candidate at a time. Used when a subunit's
+// neighbourhood overflows the LDS records of the fast build.
+__device__ void build_lists_exact(const Dev& d, const RepParams& P, int s, int tid, double4 pi,
+                                  const int* cand, const unsigned char* cflag, int nc, double4* stage,
+                                  bool do_lj, bool do_q, int& cnt_lj, int& cnt_q) {
+  const int np = d.topo.np, nq = d.topo.nq;
+  const double box = P.box, hbox = P.hbox;
   const bool active = tid < np;
   const int gi = s * np + tid;
-  double4 pi = active ? own[tid] : make_double4(0, 0, 0, 0);
   const int ti = active ? d.topo.type[tid] : 0;
   const double si = active ? d.topo.sigma[tid] : 0.0;
   const unsigned long long mask_i = active ? d.topo.att_mask[ti] : 0ull;
@@ -193,8 +223,8 @@ __device__ void build_lists(const Dev& d, const RepParams& P, RepState& S, int s
   const int gq = qi >= 0 ? s * nq + qi : -1;
   const double ecut_l = P.ecut + P.skin;
   const double ecut_l2 = ecut_l * ecut_l;
-  cnt_lj = 0;
-  cnt_q = 0;
+  if (do_lj) cnt_lj = 0;
+  if (do_q) cnt_q = 0;
   for (int c = 0; c < nc; c++) {
     const int t = cand[c];
     const int f = cflag[c];
@@ -202,7 +232,7 @@ __device__ void build_lists(const Dev& d, const RepParams& P, RepState& S, int s
     if (tid < np) stage[tid] = d.pos[(size_t)t * np + tid];
     __syncthreads();
     if (!active) continue;
-    if (f & 1) {
+    if (do_lj && (f & 1)) {
       for (int k = 0; k < np; k++) {
         const double4 pj = stage[k];
         const double dx = min_img(pi.x - pj.x, box, hbox), dy = min_img(pi.y - pj.y, box, hbox),
@@ -222,7 +252,7 @@ __device__ void build_lists(const Dev& d, const RepParams& P, RepState& S, int s
         }
       }
     }
-    if ((f & 2) && qi >= 0) {
+    if (do_q && (f & 2) && qi >= 0) {
       for (int kq = 0; kq < nq; kq++) {
         const int k = d.topo.qlocal[kq];
         const int j = t * np + k;
@@ -230,36 +260,272 @@ __device__ void build_lists(const Dev& d, const RepParams& P, RepState& S, int s
         const double4 pj = stage[k];
         const double dx = min_img(pi.x - pj.x, box, hbox), dy = min_img(pi.y - pj.y, box, hbox),
                      dz = min_img(pi.z - pj.z, box, hbox);
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 < P.rc_q_list2) {
+        if (dx * dx + dy * dy + dz * dz < P.rc_q_list2) {
           if (cnt_q < d.q_cap) d.q_list[(size_t)cnt_q * d.Nq + gq] = j;
           cnt_q++;
         }
       }
     }
   }
+}
+
+// j mod np and j / np for j < 2^23 via a float reciprocal and one correction.
+__device__ __forceinline__ int local_of(int j, int np, float inv_np) {
+  int q = (int)((float)j * inv_np);
+  int r = j - q * np;
+  if (r < 0) r += np;
+  else if (r >= np) r -= np;
+  return r;
+}
+
+// Fast build. Candidate beads are taken relative to the centre bead of s (minimum image,
+// then FP32). LJ candidates inside the cube |rel| <= r_bound + rl_lj are counted and placed
+// into an LDS grid of at most 8^3 cells of edge >= rl_lj; each bead of s then tests the 27
+// cells around it and keeps hits in a per-thread LDS buffer sorted by id. Yukawa candidates'
+// charged beads are staged in candidate order and scanned in order by the charged beads.
+// FP32 tests use the list radius plus 1e-3 sigma, so the lists are supersets of the exact
+// FP64 ones; entries beyond a force cutoff contribute nothing.
+__global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
+  __shared__ int cand[kMaxCand];
+  __shared__ unsigned char cflag[kMaxCand];
+  __shared__ int scan[kBlock];
+  __shared__ float4 rec[kRbRecords];
+  __shared__ int cell_cnt[kRbGrid * kRbGrid * kRbGrid];
+  __shared__ int cell_off[kRbGrid * kRbGrid * kRbGrid + 1];
+  __shared__ int ljbuf[kRbLjBuf * kBlock];
+  __shared__ int fallback_sh;
+  __shared__ float tsig[kBlock];
+  __shared__ int ttype[kBlock];
+  double4* stage = reinterpret_cast<double4*>(rec);  // exact fallback and final sums reuse the records
+
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const int r = d.sub_rep[s];
+  RepState& S = d.rs[r];
+  if (!force_rebuild && S.rebuild == 0) return;
+  const RepParams& P = d.rp[r];
+  const int np = d.topo.np, nq = d.topo.nq;
+  const double box = P.box, hbox = P.hbox;
+  const float boxf = (float)box, hboxf = (float)hbox;
+  const double4 cs = d.centres[s];
+  const bool active = tid < np;
+  const int gi = s * np + tid;
+  const double4 pi = active ? d.pos[gi] : make_double4(0, 0, 0, 0);
+  if (tid == 0) fallback_sh = P.grid_ok ? 0 : 1;
+
+  const int nc = find_candidates(d, P, S, s, tid, cs, cand, cflag, scan);
+
+  // ---- LJ: local cell grid
+  const float H = (float)(P.r_bound + P.rl_lj) + 1e-3f;
+  int G = (int)floorf(2.0f * H / ((float)P.rl_lj + 1e-3f));
+  if (G < 1) G = 1;
+  if (G > kRbGrid) G = kRbGrid;
+  const float a = 2.0f * H / (float)G;
+  const float inv_a = 1.0f / a;
+  const int ncell = G * G * G;
+  for (int c = tid; c < ncell; c += kBlock) cell_cnt[c] = 0;
+  if (active) { tsig[tid] = (float)d.topo.sigma[tid]; ttype[tid] = d.topo.type[tid]; }
+  __syncthreads();
+  // pass 1: count (the beads of kPipe candidates are fetched together)
+  for (int c0 = 0; c0 < nc; c0 += kRbPipe) {
+    double4 pj[kRbPipe];
+    bool use[kRbPipe];
+#pragma unroll
+    for (int u = 0; u < kRbPipe; u++) {
+      use[u] = active && c0 + u < nc && (cflag[c0 + u] & 1);
+      if (use[u]) pj[u] = d.pos[(size_t)cand[c0 + u] * np + tid];
+    }
+#pragma unroll
+    for (int u = 0; u < kRbPipe; u++) {
+      if (!use[u]) continue;
+      const float rx = (float)min_img(pj[u].x - cs.x, box, hbox), ry = (float)min_img(pj[u].y - cs.y, box, hbox),
+                  rz = (float)min_img(pj[u].z - cs.z, box, hbox);
+      if (fabsf(rx) <= H && fabsf(ry) <= H && fabsf(rz) <= H) {
+        const int cx = min(G - 1, (int)((rx + H) * inv_a)), cy = min(G - 1, (int)((ry + H) * inv_a)),
+                  cz = min(G - 1, (int)((rz + H) * inv_a));
+        atomicAdd(&cell_cnt[(cz * G + cy) * G + cx], 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int c = 0; c < ncell; c++) { cell_off[c] = acc; acc += cell_cnt[c]; }
+    cell_off[ncell] = acc;
+    if (acc > kRbRecords) fallback_sh = 1;
+  }
+  __syncthreads();
+  for (int c = tid; c < ncell; c += kBlock) cell_cnt[c] = 0;
+  __syncthreads();
+  const bool fast = fallback_sh == 0;
+  // pass 2: place
+  if (fast) {
+    for (int c0 = 0; c0 < nc; c0 += kRbPipe) {
+      double4 pj[kRbPipe];
+      bool use[kRbPipe];
+#pragma unroll
+      for (int u = 0; u < kRbPipe; u++) {
+        use[u] = active && c0 + u < nc && (cflag[c0 + u] & 1);
+        if (use[u]) pj[u] = d.pos[(size_t)cand[c0 + u] * np + tid];
+      }
+#pragma unroll
+      for (int u = 0; u < kRbPipe; u++) {
+        if (!use[u]) continue;
+        const int j = cand[c0 + u] * np + tid;
+        const float rx = (float)min_img(pj[u].x - cs.x, box, hbox), ry = (float)min_img(pj[u].y - cs.y, box, hbox),
+                    rz = (float)min_img(pj[u].z - cs.z, box, hbox);
+        if (fabsf(rx) <= H && fabsf(ry) <= H && fabsf(rz) <= H) {
+          const int cx = min(G - 1, (int)((rx + H) * inv_a)), cy = min(G - 1, (int)((ry + H) * inv_a)),
+                    cz = min(G - 1, (int)((rz + H) * inv_a));
+          const int cell = (cz * G + cy) * G + cx;
+          const int slot = cell_off[cell] + atomicAdd(&cell_cnt[cell], 1);
+          rec[slot] = make_float4(rx, ry, rz, __int_as_float(j));
+        }
+      }
+    }
+  }
+  __syncthreads();
+  int cnt_lj = 0, cnt_q = 0;
+  if (fast && active) {
+    const int ti = d.topo.type[tid];
+    const float si = (float)d.topo.sigma[tid];
+    const unsigned long long mask_i = d.topo.att_mask[ti];
+    const float ratt = (float)(P.ecut + P.skin) + 1e-3f;
+    const float ratt2 = ratt * ratt;
+    const float skinf = (float)P.skin + 1e-3f;
+    const float inv_np = 1.0f / (float)np;
+    const float ix = (float)min_img(pi.x - cs.x, box, hbox), iy = (float)min_img(pi.y - cs.y, box, hbox),
+                iz = (float)min_img(pi.z - cs.z, box, hbox);
+    const int cx = min(G - 1, max(0, (int)((ix + H) * inv_a))), cy = min(G - 1, max(0, (int)((iy + H) * inv_a))),
+              cz = min(G - 1, max(0, (int)((iz + H) * inv_a)));
+    bool over = false;
+    for (int zz = max(0, cz - 1); zz <= min(G - 1, cz + 1); zz++)
+      for (int yy = max(0, cy - 1); yy <= min(G - 1, cy + 1); yy++)
+        for (int xx = max(0, cx - 1); xx <= min(G - 1, cx + 1); xx++) {
+          const int cell = (zz * G + yy) * G + xx;
+          for (int m = cell_off[cell]; m < cell_off[cell + 1]; m++) {
+            const float4 rj = rec[m];
+            const int j = __float_as_int(rj.w);
+            const int lj = local_of(j, np, inv_np);
+            const int tj = ttype[lj];
+            const float dx = min_imgf(ix - rj.x, boxf, hboxf), dy = min_imgf(iy - rj.y, boxf, hboxf),
+                        dz = min_imgf(iz - rj.z, boxf, hboxf);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            float thr2;
+            if ((mask_i >> tj) & 1ull) {
+              thr2 = ratt2;
+            } else {
+              const float w = (float)kWcaFactor * 0.5f * (si + tsig[lj]) + skinf;
+              thr2 = w * w;
+            }
+            if (r2 < thr2) {
+              if (cnt_lj < kRbLjBuf) {
+                int p = cnt_lj;
+                while (p > 0 && ljbuf[(p - 1) * kBlock + tid] > j) {
+                  ljbuf[p * kBlock + tid] = ljbuf[(p - 1) * kBlock + tid];
+                  p--;
+                }
+                ljbuf[p * kBlock + tid] = j;
+              } else {
+                over = true;
+              }
+              cnt_lj++;
+            }
+          }
+        }
+    if (over) fallback_sh = 1;
+  }
+  __syncthreads();
+  const bool lj_fast = fallback_sh == 0;
+  if (lj_fast && active) {
+    for (int k = 0; k < cnt_lj && k < d.lj_cap; k++) d.lj_list[(size_t)k * d.Nb + gi] = ljbuf[k * kBlock + tid];
+  }
+  // ---- Yukawa: charged beads of the candidates, in candidate order
+  bool q_fast = true;
+  if (nq > 0) {
+    int nqc = 0;  // Yukawa candidates are a prefix-ordered subset of cand
+    __syncthreads();
+    if (tid == 0) {
+      int k = 0;
+      for (int c = 0; c < nc; c++)
+        if (cflag[c] & 2) {
+          if (k < kRbGrid * kRbGrid * kRbGrid) cell_cnt[k] = cand[c];  // reuse cell_cnt as the ordered list
+          k++;
+        }
+      cell_off[0] = k;
+    }
+    __syncthreads();
+    nqc = cell_off[0];
+    const int nrec = nqc * nq;
+    q_fast = nrec <= kRbRecords && nqc <= kRbGrid * kRbGrid * kRbGrid;
+    if (q_fast) {
+      for (int f = tid; f < nrec; f += kBlock) {
+        const int c = f / nq, kq = f - c * nq;
+        const int j = cell_cnt[c] * np + d.topo.qlocal[kq];
+        const double4 pj = d.pos[j];
+        rec[f] = make_float4((float)min_img(pj.x - cs.x, box, hbox), (float)min_img(pj.y - cs.y, box, hbox),
+                             (float)min_img(pj.z - cs.z, box, hbox), __int_as_float(j));
+      }
+      __syncthreads();
+      const int qi = active ? d.topo.qslot[tid] : -1;
+      if (qi >= 0) {
+        const int gq = s * nq + qi;
+        const float rq = sqrtf((float)P.rc_q_list2) + 1e-3f;
+        const float rq2 = rq * rq;
+        const float ix = (float)min_img(pi.x - cs.x, box, hbox), iy = (float)min_img(pi.y - cs.y, box, hbox),
+                    iz = (float)min_img(pi.z - cs.z, box, hbox);
+        for (int f = 0; f < nrec; f++) {
+          const float4 rj = rec[f];
+          const int j = __float_as_int(rj.w);
+          if (j == gi) continue;
+          const float dx = min_imgf(ix - rj.x, boxf, hboxf), dy = min_imgf(iy - rj.y, boxf, hboxf),
+                      dz = min_imgf(iz - rj.z, boxf, hboxf);
+          if (dx * dx + dy * dy + dz * dz < rq2) {
+            if (cnt_q < d.q_cap) d.q_list[(size_t)cnt_q * d.Nq + gq] = j;
+            cnt_q++;
+          }
+        }
+      }
+    }
+  }
+  // ---- exact fallback for whatever overflowed the fast path (uniform branches)
+  if (!lj_fast || !q_fast) {
+    __syncthreads();
+    build_lists_exact(d, P, s, tid, pi, cand, cflag, nc, stage, !lj_fast, !q_fast, cnt_lj, cnt_q);
+  }
   if (active) {
+    const int qi = d.topo.qslot[tid];
     d.lj_cnt[gi] = cnt_lj;
-    if (gq >= 0) d.q_cnt[gq] = cnt_q;
+    if (qi >= 0) d.q_cnt[s * nq + qi] = cnt_q;
     d.xb[gi] = pi;
     if (cnt_lj > d.lj_cap) atomicMax(&S.need_lj, cnt_lj);
     if (cnt_q > d.q_cap) atomicMax(&S.need_q, cnt_q);
-    const double4 pc = own[d.topo.centre];
-    const double dx = min_img(pi.x - pc.x, box, hbox), dy = min_img(pi.y - pc.y, box, hbox),
-                 dz = min_img(pi.z - pc.z, box, hbox);
-    if (dx * dx + dy * dy + dz * dz > P.r_bound2) S.deformed = 1;
+    const double dx = min_img(pi.x - cs.x, box, hbox), dy = min_img(pi.y - cs.y, box, hbox),
+                 dz = min_img(pi.z - cs.z, box, hbox);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > P.r_bound2 || (qi >= 0 && d2 > P.r_bound_q2)) S.deformed = 1;
+  }
+  // list sizes of this subunit for the statistics
+  const double wl = wave_sum(active ? (double)cnt_lj : 0.0), wq = wave_sum(active ? (double)cnt_q : 0.0);
+  __syncthreads();
+  double* red = reinterpret_cast<double*>(stage);
+  if ((tid & 63) == 0) { red[2 * (tid >> 6)] = wl; red[2 * (tid >> 6) + 1] = wq; }
+  __syncthreads();
+  if (tid == 0) {
+    d.part[(size_t)s * kPartStride + 7] = red[0] + red[2];
+    d.part[(size_t)s * kPartStride + 8] = red[1] + red[3];
   }
 }
 
+// ------------------------------------------------------------------------------------------
 // Bending of one hinge: E = Kb (1 - |cos t|), cos t the cosine between the normals of the two
 // faces; equal to the reference's (grad S - (S/a0 grad a0 + S/a1 grad a1)) Kb / (4 a0 a1) with
 // S = 4 |A0.A1| (src/edge.cpp:69-188). With e = x1-x0, a = x2-x0, b = x3-x0, u = e x a,
 // v = e x b, c = u.v/(|u||v|): dc/du = g = (v/|v| - c u/|u|)/|u|, dc/dv = h likewise, and
 // dc/dx2 = g x e, dc/dx3 = h x e, dc/dx1 = a x g + b x h, dc/dx0 = -(sum).
 // Writes the 4 bead forces (12 doubles) and returns cos t.
-__device__ __forceinline__ double hinge_forces(const double4* own, const int* hg, double box, double hbox,
-                                               double Kb, double* out) {
-  const double4 p0 = own[hg[0]], p1 = own[hg[1]], p2 = own[hg[2]], p3 = own[hg[3]];
+__device__ __forceinline__ double hinge_forces(const double4* own, int4 hg, double box, double hbox, double Kb,
+                                               double* out) {
+  const double4 p0 = own[hg.x], p1 = own[hg.y], p2 = own[hg.z], p3 = own[hg.w];
   const double ex = min_img(p1.x - p0.x, box, hbox), ey = min_img(p1.y - p0.y, box, hbox),
                ez = min_img(p1.z - p0.z, box, hbox);
   const double ax = min_img(p2.x - p0.x, box, hbox), ay = min_img(p2.y - p0.y, box, hbox),
@@ -287,141 +553,211 @@ __device__ __forceinline__ double hinge_forces(const double4* own, const int* hg
   return c;
 }
 
-// LJ / WCA gather over bead i's list (src/forces.cpp:112-139): attractive rows use
+constexpr int kChunk = 256;  // list entries a workgroup evaluates per gather round
+
+// Per-template-bead tables a workgroup keeps in LDS for the neighbour lookups.
+struct BeadLds {
+  double sigma[kBlock];
+  int type[kBlock];
+  int qslot[kBlock];
+  unsigned long long mask[kMaxTypes];
+};
+
+// LJ / WCA term on bead i from bead j (src/forces.cpp:112-139): attractive rows use
 // 48 eps (s_i^6/r^6)((s_i^6/r^6) - 1/2)/r^2 below ecut with eps = elj_att, or 50 when
 // type_j == 7; other pairs the WCA form with eps = 1 and s = (s_i + s_j)/2 below
-// 1.12246205 s.
+// 1.12246205 s. Returns the scalar f of F = f r_vec (0 beyond the cutoff).
 template <typename PT>
-__device__ __forceinline__ void lj_gather(const Dev& d, const RepParams& P, int gi, int cnt, double4 pi,
-                                          double si, unsigned long long mask_i, PT& fx, PT& fy, PT& fz) {
-  const double box = P.box, hbox = P.hbox;
-  const PT ecut2 = (PT)P.ecut2, elj = (PT)P.elj_att;
-  const PT si_ = (PT)si;
-  const PT si6 = si_ * si_ * si_ * si_ * si_ * si_;
-  const PT replj2 = (PT)(kWcaFactor * kWcaFactor);
-  for (int k = 0; k < cnt; k++) {
-    const int j = d.lj_list[(size_t)k * d.Nb + gi];
-    const double4 pj = d.pos[j];
-    const int lj = tag_loc(pj.w);
-    const int tj = d.topo.type[lj];
-    const PT sj = (PT)d.topo.sigma[lj];
-    const PT dx = (PT)min_img(pi.x - pj.x, box, hbox), dy = (PT)min_img(pi.y - pj.y, box, hbox),
-             dz = (PT)min_img(pi.z - pj.z, box, hbox);
-    const PT r2 = dx * dx + dy * dy + dz * dz;
-    const PT shc = (si_ + sj) / (PT)2;
-    PT s6, eps;
-    bool on;
-    if ((mask_i >> tj) & 1ull) {
-      on = r2 < ecut2;
-      s6 = si6;
-      eps = (tj == 7) ? (PT)50 : elj;
-    } else {
-      on = r2 < replj2 * shc * shc;
-      s6 = shc * shc * shc * shc * shc * shc;
-      eps = (PT)1;
-    }
-    if (on) {
-      const PT r6 = r2 * r2 * r2;
-      const PT sr6 = s6 / r6;
-      const PT f = (PT)48 * eps * (sr6 * (sr6 - (PT)0.5)) * ((PT)1 / r2);
-      fx += dx * f; fy += dy * f; fz += dz * f;
-    }
+__device__ __forceinline__ PT lj_term(PT r2, bool att, int tj, PT si, PT sj, PT ecut2, PT elj) {
+  const PT shc = (si + sj) / (PT)2;
+  PT s6, eps;
+  bool on;
+  if (att) {
+    on = r2 < ecut2;
+    s6 = si * si * si * si * si * si;
+    eps = (tj == 7) ? (PT)50 : elj;
+  } else {
+    on = r2 < (PT)(kWcaFactor * kWcaFactor) * shc * shc;
+    s6 = shc * shc * shc * shc * shc * shc;
+    eps = (PT)1;
   }
+  if (!on) return (PT)0;
+  const PT r6 = r2 * r2 * r2;
+  const PT sr6 = s6 / r6;
+  return (PT)48 * eps * (sr6 * (sr6 - (PT)0.5)) * ((PT)1 / r2);
 }
 
-// Screened Coulomb gather over charged bead i's list (src/forces.cpp:100-108):
-// q_i q_j lb exp(-k r)/r^2 (k + 1/r) Y_ij r_vec below ecut_el.
+// Screened Coulomb term (src/forces.cpp:100-108): q_i q_j lb exp(-k r)/r^2 (k + 1/r) Y_ij below
+// ecut_el; returns f of F = f r_vec.
 template <typename PT>
-__device__ __forceinline__ void yk_gather(const Dev& d, const RepParams& P, int gq, int qi, int cnt, double4 pi,
-                                          PT& fx, PT& fy, PT& fz) {
-  const double box = P.box, hbox = P.hbox;
-  const int nq = d.topo.nq;
-  const PT kappa = (PT)P.kappa, ecut_el2 = (PT)P.ecut_el2;
-  for (int k = 0; k < cnt; k++) {
-    const int j = d.q_list[(size_t)k * d.Nq + gq];
-    const double4 pj = d.pos[j];
-    const int qj = d.topo.qslot[tag_loc(pj.w)];
-    const PT dx = (PT)min_img(pi.x - pj.x, box, hbox), dy = (PT)min_img(pi.y - pj.y, box, hbox),
-             dz = (PT)min_img(pi.z - pj.z, box, hbox);
-    const PT r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < ecut_el2) {
-      const PT r = sqrt(r2);
-      const PT f = ((PT)P.qq[qi * nq + qj] * exp(-kappa * r) / r2) * (kappa + (PT)1 / r) * (PT)P.yk[qi * nq + qj];
-      fx += dx * f; fy += dy * f; fz += dz * f;
-    }
-  }
+__device__ __forceinline__ PT yk_term(PT r2, PT qq, PT yk, PT kappa, PT ecut_el2) {
+  if (!(r2 < ecut_el2)) return (PT)0;
+  const PT rr = sqrt(r2);
+  return (qq * exp(-kappa * rr) / r2) * (kappa + (PT)1 / rr) * yk;
+}
+
+// LDS bytes of k_force: own positions, hinge forces, Yukawa tables of the charged beads, and
+// the pair terms of one gather round.
+static inline size_t force_lds_bytes(int np, int nh, int nq) {
+  return (size_t)np * 32 + (size_t)nh * 96 + (size_t)nq * nq * 16 + (size_t)kChunk * 3 * 8;
 }
 
 template <int MODE, typename PT>
-__global__ __launch_bounds__(kBlock) void k_force(Dev d, int init, int force_rebuild) {
+__global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
   extern __shared__ double smem[];
-  __shared__ int cand[kMaxCand];
-  __shared__ unsigned char cflag[kMaxCand];
-  __shared__ int wcnt[2];
-  __shared__ int ncand_sh;
   __shared__ double red[2][kPartStride];
+  __shared__ BeadLds tb;
+  __shared__ int off_s[kBlock + 1];
+  __shared__ int nlj_s[kBlock];
+  __shared__ unsigned char owner_s[kChunk];
   const int np = d.topo.np, nh = d.topo.nh, nq = d.topo.nq;
   double4* own = reinterpret_cast<double4*>(smem);
-  double4* stage = own + np;
-  double* hc = reinterpret_cast<double*>(stage + np);
+  double* hc = reinterpret_cast<double*>(own + np);
+  double* qq_s = hc + 12 * nh;
+  double* yk_s = qq_s + nq * nq;
+  PT* pf = reinterpret_cast<PT*>(yk_s + nq * nq);  // [3][kChunk]
 
   const int s = blockIdx.x, tid = threadIdx.x;
   const int r = d.sub_rep[s];
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
-  const double box = P.box, hbox = P.hbox;
+  const double box = P.box, hbox = P.hbox, kb = P.kb, ks = P.ks, dt = P.dt;
   const bool active = tid < np;
   const int gi = s * np + tid;
-  if (active) own[tid] = d.pos[gi];
-  const bool rebuild = force_rebuild || S.rebuild != 0;
-  __syncthreads();
-
-  int cnt_lj = 0, cnt_q = 0;
-  const int qi = active ? d.topo.qslot[tid] : -1;
-  const int gq = qi >= 0 ? s * nq + qi : -1;
-  if (rebuild) {
-    build_lists(d, P, S, s, tid, own, stage, cand, cflag, wcnt, &ncand_sh, cnt_lj, cnt_q);
-  } else if (active) {
-    cnt_lj = d.lj_cnt[gi];
-    if (gq >= 0) cnt_q = d.q_cnt[gq];
+  const int lt = active ? tid : 0;
+  // one round of independent loads: own position, velocity, list sizes, the packed edge and
+  // hinge rows, the template bead tables and the Yukawa tables
+  int qi = -1, nlj = 0, nqe = 0;
+  if (active) {
+    own[tid] = d.pos[gi];
+    tb.sigma[tid] = d.topo.sigma[tid];
+    tb.type[tid] = d.topo.type[tid];
+    qi = d.topo.qslot[tid];
+    tb.qslot[tid] = qi;
+    nlj = min(d.lj_cnt[gi], d.lj_cap);
+    if (qi >= 0) nqe = min(d.q_cnt[s * nq + qi], d.q_cap);
   }
+  if (tid < kMaxTypes) tb.mask[tid] = d.topo.att_mask[tid];
+  for (int k = tid; k < nq * nq; k += kBlock) { qq_s[k] = P.qq[k]; yk_s[k] = P.yk[k]; }
+  const int ecnt = d.topo.e_cnt[lt];
+  const int hcnt = d.topo.h_cnt[lt];
+  const int2 erow = reinterpret_cast<const int2*>(d.topo.e_other)[lt];
+  const double2* lrow = reinterpret_cast<const double2*>(d.topo.e_len0) + (size_t)lt * (kMaxDeg / 2);
+  double2 len[kMaxDeg / 2];
+#pragma unroll
+  for (int k = 0; k < kMaxDeg / 2; k++) len[k] = lrow[k];
+  const int4 hrow0 = reinterpret_cast<const int4*>(d.topo.h_slot)[2 * lt];
+  const int4 hrow1 = reinterpret_cast<const int4*>(d.topo.h_slot)[2 * lt + 1];
+  const double m = d.topo.mass[lt];
+  double vx = 0, vy = 0, vz = 0;
+  if (active) { vx = d.vx[gi]; vy = d.vy[gi]; vz = d.vz[gi]; }
+  // offsets of each bead's entries in the subunit's concatenated lists (LJ entries, then
+  // Yukawa entries, bead by bead): wave prefix sums, then the two waves in order
+  const int n_i = nlj + nqe;
+  int incl = n_i;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if ((tid & 63) >= o) incl += v;
+  }
+  if ((tid & 63) == 63) off_s[kBlock - 1 - (tid >> 6)] = incl;  // wave totals, parked at the end
+  __syncthreads();
+  const int wave0_total = off_s[kBlock - 1];
+  const int off_i = incl - n_i + ((tid >> 6) ? wave0_total : 0);
+  const int total = wave0_total + off_s[kBlock - 2];
+  __syncthreads();
+  off_s[tid] = off_i;
+  nlj_s[tid] = nlj;
+  if (tid == 0) off_s[kBlock] = total;
   // bending hinges -> LDS
-  for (int h = tid; h < nh; h += kBlock) hinge_forces(own, d.topo.hinge + 4 * h, box, hbox, P.kb, hc + 12 * h);
+  for (int h = tid; h < nh; h += kBlock) {
+    const int4 hg = reinterpret_cast<const int4*>(d.topo.hinge)[h];
+    hinge_forces(own, hg, box, hbox, kb, hc + 12 * h);
+  }
+  // stretching (src/bead.cpp:17-23) over the bead's edge row
+  double sx = 0, sy = 0, sz = 0;
+  double4 pi = make_double4(0, 0, 0, 0);
+  if (active) {
+    pi = own[tid];
+#pragma unroll
+    for (int k = 0; k < kMaxDeg; k++) {
+      if (k >= ecnt) break;
+      const int o = ((k < 4 ? erow.x : erow.y) >> (8 * (k & 3))) & 0xff;
+      const double l0 = (k & 1) ? len[k >> 1].y : len[k >> 1].x;
+      const double4 po = own[o];
+      const double dx = min_img(pi.x - po.x, box, hbox), dy = min_img(pi.y - po.y, box, hbox),
+                   dz = min_img(pi.z - po.z, box, hbox);
+      const double L = sqrt(dx * dx + dy * dy + dz * dz);
+      const double c = (l0 - L) * ks / L;
+      sx += dx * c; sy += dy * c; sz += dz * c;
+    }
+  }
+  // pair terms: all threads evaluate the subunit's list entries kChunk at a time; each bead
+  // then adds its own entries in list order, the sum of a sequential per-bead loop
+  PT lx = 0, ly = 0, lz = 0, ex = 0, ey = 0, ez = 0;
+  {
+    const PT ecut2 = (PT)P.ecut2, elj = (PT)P.elj_att, kappa = (PT)P.kappa, ecut_el2 = (PT)P.ecut_el2;
+    for (int c0 = 0; c0 < total; c0 += kChunk) {
+      __syncthreads();
+      const int c1 = min(total, c0 + kChunk);
+      if (active)
+        for (int e = max(off_i, c0); e < min(off_i + n_i, c1); e++) owner_s[e - c0] = (unsigned char)tid;
+      __syncthreads();
+      for (int e = c0 + tid; e < c1; e += kBlock) {
+        const int i = owner_s[e - c0];
+        const int k = e - off_s[i];
+        const int nl = nlj_s[i];
+        const int bi = s * np + i;
+        const int qs = tb.qslot[i];
+        const int j = (k < nl) ? d.lj_list[(size_t)k * d.Nb + bi] : d.q_list[(size_t)(k - nl) * d.Nq + (s * nq + qs)];
+        const double4 pj = d.pos[j];
+        const double4 p0 = own[i];
+        const PT dx = (PT)min_img(p0.x - pj.x, box, hbox), dy = (PT)min_img(p0.y - pj.y, box, hbox),
+                 dz = (PT)min_img(p0.z - pj.z, box, hbox);
+        const PT r2 = dx * dx + dy * dy + dz * dz;
+        const int lj = tag_loc(pj.w);
+        PT f;
+        if (k < nl) {
+          const int tj = tb.type[lj];
+          f = lj_term<PT>(r2, (tb.mask[tb.type[i]] >> tj) & 1ull, tj, (PT)tb.sigma[i], (PT)tb.sigma[lj], ecut2, elj);
+        } else {
+          const int qj = tb.qslot[lj];
+          f = yk_term<PT>(r2, (PT)qq_s[qs * nq + qj], (PT)yk_s[qs * nq + qj], kappa, ecut_el2);
+        }
+        pf[e - c0] = dx * f;
+        pf[kChunk + e - c0] = dy * f;
+        pf[2 * kChunk + e - c0] = dz * f;
+      }
+      __syncthreads();
+      if (active) {
+        const int e1 = min(off_i + n_i, c1);
+        for (int e = max(off_i, c0); e < e1; e++) {
+          const PT fx = pf[e - c0], fy = pf[kChunk + e - c0], fz = pf[2 * kChunk + e - c0];
+          if (e - off_i < nlj) { lx += fx; ly += fy; lz += fz; }
+          else { ex += fx; ey += fy; ez += fz; }
+        }
+      }
+    }
+  }
   __syncthreads();
 
   double part[kPartStride];
 #pragma unroll
   for (int k = 0; k < kPartStride; k++) part[k] = 0.0;
   if (active) {
-    const double4 pi = own[tid];
-    // stretching (src/bead.cpp:17-23)
-    double sx = 0, sy = 0, sz = 0;
-    for (int k = d.topo.e_off[tid]; k < d.topo.e_off[tid + 1]; k++) {
-      const double4 po = own[d.topo.e_other[k]];
-      const double dx = min_img(pi.x - po.x, box, hbox), dy = min_img(pi.y - po.y, box, hbox),
-                   dz = min_img(pi.z - po.z, box, hbox);
-      const double L = sqrt(dx * dx + dy * dy + dz * dz);
-      const double c = (d.topo.e_len0[k] - L) * P.ks / L;
-      sx += dx * c; sy += dy * c; sz += dz * c;
-    }
     // bending gather (fixed hinge order)
     double bx = 0, by = 0, bz = 0;
-    for (int k = d.topo.h_off[tid]; k < d.topo.h_off[tid + 1]; k++) {
-      const int sl = d.topo.h_slot[k];
+    const int hw[8] = {hrow0.x, hrow0.y, hrow0.z, hrow0.w, hrow1.x, hrow1.y, hrow1.z, hrow1.w};
+#pragma unroll
+    for (int k = 0; k < kMaxHinc; k++) {
+      if (k >= hcnt) break;
+      const int sl = (hw[k >> 1] >> (16 * (k & 1))) & 0xffff;
       const double* q = hc + 12 * (sl >> 2) + 3 * (sl & 3);
       bx += q[0]; by += q[1]; bz += q[2];
     }
-    // pairs
-    const double si = d.topo.sigma[tid];
-    const unsigned long long mask_i = d.topo.att_mask[d.topo.type[tid]];
-    PT lx = 0, ly = 0, lz = 0, ex = 0, ey = 0, ez = 0;
-    lj_gather<PT>(d, P, gi, min(cnt_lj, d.lj_cap), pi, si, mask_i, lx, ly, lz);
-    if (gq >= 0) yk_gather<PT>(d, P, gq, qi, min(cnt_q, d.q_cap), pi, ex, ey, ez);
     // src/forces.cpp:141-144: sforce + bforce + (eForce + ljForce)
     const double elx = (double)ex + (double)lx, ely = (double)ey + (double)ly, elz = (double)ez + (double)lz;
     double Fx = sx + bx + elx, Fy = sy + by + ely, Fz = sz + bz + elz;
-    const double m = d.topo.mass[tid];
-    double vx = d.vx[gi], vy = d.vy[gi], vz = d.vz[gi];
     if (MODE == MODE_FORCES) {
       if (d.out_f) { d.out_f[3 * (size_t)gi] = Fx; d.out_f[3 * (size_t)gi + 1] = Fy; d.out_f[3 * (size_t)gi + 2] = Fz; }
       if (d.out_s) { d.out_s[3 * (size_t)gi] = sx; d.out_s[3 * (size_t)gi + 1] = sy; d.out_s[3 * (size_t)gi + 2] = sz; }
@@ -437,13 +773,12 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init, int force_reb
       d.fx[gi] = Fx; d.fy[gi] = Fy; d.fz[gi] = Fz;
     } else {
       // BD: tforce = F + (fran - mean) + fdrag; v += tforce dt/(2m) (src/md.cpp:462-498)
-      const int bl = gi - P.b0;
       double ux, uy, uz;
-      bd_draws(S.rng, bl, ux, uy, uz);
+      bd_draws(S.rng, gi - P.b0, ux, uy, uz);
       const double amp = P.bd_amp[tid];
       const double rx = amp * ux - S.mean_fran[0], ry = amp * uy - S.mean_fran[1], rz = amp * uz - S.mean_fran[2];
       Fx = Fx + rx + d.gx[gi]; Fy = Fy + ry + d.gy[gi]; Fz = Fz + rz + d.gz[gi];
-      const double c = 0.5 * P.dt / m;
+      const double c = 0.5 * dt / m;
       vx = vx + Fx * c; vy = vy + Fy * c; vz = vz + Fz * c;
       d.vx[gi] = vx; d.vy[gi] = vy; d.vz[gi] = vz;
       d.fx[gi] = Fx; d.fy[gi] = Fy; d.fz[gi] = Fz;
@@ -458,56 +793,101 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init, int force_reb
       const double amp = P.bd_amp[tid];
       part[4] = amp * ux; part[5] = amp * uy; part[6] = amp * uz;
     }
-    if (rebuild) { part[7] = (double)cnt_lj; part[8] = (double)cnt_q; }
   }
-  // per-subunit sums, fixed shape: wave shuffles then the two waves in order
+  // per-subunit sums, fixed shape: wave shuffles then the two waves in order (slots 7-8 are
+  // the list sizes written by k_rebuild)
 #pragma unroll
-  for (int k = 0; k < kPartStride; k++) {
+  for (int k = 0; k < 7; k++) {
     const double w = wave_sum(part[k]);
     if ((tid & 63) == 0) red[tid >> 6][k] = w;
   }
   __syncthreads();
-  if (tid < kPartStride) d.part[(size_t)s * kPartStride + tid] = red[0][tid] + red[1][tid];
+  if (tid < 7) d.part[(size_t)s * kPartStride + tid] = red[0][tid] + red[1][tid];
 }
 
 // One workgroup per replica: fixed-order sums of the per-subunit partials, then the scalar
 // bookkeeping of src/md.cpp:400-406, 456-460, 469-478, 506-514.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_finalize(Dev d, int init, int force_rebuild) {
-  __shared__ double red[kPartStride][256];
+constexpr int kFinThreads = 1024;
+
+__global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
+  __shared__ double wred[kFinThreads / 64][kPartStride];
+  __shared__ double tot[kPartStride];
   const int r = blockIdx.x, tid = threadIdx.x;
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
+  double acc[kPartStride];
+#pragma unroll
+  for (int k = 0; k < kPartStride; k++) acc[k] = 0.0;
+  const int s_end = P.s0 + P.ns;
+  for (int s0 = P.s0 + tid; s0 < s_end; s0 += 2 * kFinThreads) {
+    double v[2][kPartStride];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int s = s0 + u * kFinThreads;
+      const double2* src = reinterpret_cast<const double2*>(d.part + (size_t)s * kPartStride);
+#pragma unroll
+      for (int k = 0; k < kPartStride / 2; k++) {
+        const double2 w = (s < s_end) ? src[k] : make_double2(0.0, 0.0);
+        v[u][2 * k] = w.x; v[u][2 * k + 1] = w.y;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int k = 0; k < kPartStride; k++) acc[k] += v[u][k];
+  }
+#pragma unroll
   for (int k = 0; k < kPartStride; k++) {
-    double acc = 0.0;
-    for (int s = P.s0 + tid; s < P.s0 + P.ns; s += 256) acc += d.part[(size_t)s * kPartStride + k];
-    red[k][tid] = acc;
+    const double w = wave_sum(acc[k]);
+    if ((tid & 63) == 0) wred[tid >> 6][k] = w;
   }
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (tid < o)
-      for (int k = 0; k < kPartStride; k++) red[k][tid] += red[k][tid + o];
-    __syncthreads();
+  if (tid < kPartStride) {
+    double t = 0.0;
+    for (int w = 0; w < kFinThreads / 64; w++) t += wred[w][tid];
+    tot[tid] = t;
   }
+  __syncthreads();
   if (tid != 0) return;
+  double red[kPartStride][1];
+#pragma unroll
+  for (int k = 0; k < kPartStride; k++) red[k][0] = tot[k];
   const double ke = red[0][0];
   const int q = P.q_chain;
-  const double dt = P.dt;
+  const double dt = P.dt, T = P.T;
   if (!P.brownian) {
-    if (init) {
-      for (int j = 0; j < kMaxChain; j++) { S.xi[j] = 0; S.eta[j] = 0; }
-    } else {
-      // end of this step: eta update, forward chain with the new kinetic energy
-      for (int j = 0; j < q; j++) { S.xi[j] = S.xi_n[j]; S.eta[j] = S.eta_n[j]; }
-      for (int j = 0; j < q; j++) if (P.Qb[j] != 0) S.eta[j] = S.eta[j] + 0.5 * dt * S.xi[j];
-      for (int j = 0; j < q; j++) chain_xi(j, S.xi, P.Qb, P.dof, P.T, dt, ke);
+    // the chain lives in registers for the scalar recurrence (loops unrolled over kMaxChain)
+    double Qb[kMaxChain], dof[kMaxChain], xi[kMaxChain + 1], eta[kMaxChain];
+#pragma unroll
+    for (int j = 0; j < kMaxChain; j++) {
+      Qb[j] = P.Qb[j]; dof[j] = P.dof[j];
+      xi[j] = init ? 0.0 : S.xi_n[j];
+      eta[j] = init ? 0.0 : S.eta_n[j];
     }
+    xi[kMaxChain] = 0.0;
+    if (!init) {
+      // end of this step: eta update, forward chain with the new kinetic energy
+#pragma unroll
+      for (int j = 0; j < kMaxChain; j++)
+        if (j < q && Qb[j] != 0) eta[j] = eta[j] + 0.5 * dt * xi[j];
+#pragma unroll
+      for (int j = 0; j < kMaxChain; j++)
+        if (j < q) chain_xi(j, xi, Qb, dof, T, dt, ke);
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxChain; j++) { S.xi[j] = xi[j]; S.eta[j] = eta[j]; }
     // first thermostat half of the next step: reverse chain, eta update
-    for (int j = 0; j < kMaxChain; j++) { S.xi_n[j] = S.xi[j]; S.eta_n[j] = S.eta[j]; }
-    for (int j = q - 1; j >= 0; j--) chain_xi(j, S.xi_n, P.Qb, P.dof, P.T, dt, ke);
-    for (int j = 0; j < q; j++) if (P.Qb[j] != 0) S.eta_n[j] = S.eta_n[j] + 0.5 * dt * S.xi_n[j];
-    S.expfac = exp(-0.5 * dt * S.xi_n[0]);
-    S.kick_c = 0.5 * dt * sqrt(S.expfac);
+#pragma unroll
+    for (int j = kMaxChain - 1; j >= 0; j--)
+      if (j < q) chain_xi(j, xi, Qb, dof, T, dt, ke);
+#pragma unroll
+    for (int j = 0; j < kMaxChain; j++)
+      if (j < q && Qb[j] != 0) eta[j] = eta[j] + 0.5 * dt * xi[j];
+#pragma unroll
+    for (int j = 0; j < kMaxChain; j++) { S.xi_n[j] = xi[j]; S.eta_n[j] = eta[j]; }
+    const double ef = exp(-0.5 * dt * xi[0]);
+    S.expfac = ef;
+    S.kick_c = 0.5 * dt * sqrt(ef);
   }
   S.ke = ke;
   if (init) {
@@ -522,7 +902,7 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d, int init, int force_reb
     S.rng = init ? P.seed : lcg_mulmod(S.rng, P.mult6n);
     S.mean_fran[0] = red[4][0] * P.inv_nb; S.mean_fran[1] = red[5][0] * P.inv_nb; S.mean_fran[2] = red[6][0] * P.inv_nb;
   }
-  if (force_rebuild || S.rebuild) {
+  if (init || S.rebuild) {
     if (!init) S.rebuilds += 1;
     S.lj_entries = (long long)red[7][0];
     S.q_entries = (long long)red[8][0];
@@ -561,22 +941,25 @@ __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
   __syncthreads();
   for (int h = tid; h < nh; h += kBlock) {
     double tmp[12];
-    const double c = hinge_forces(own, d.topo.hinge + 4 * h, box, hbox, P.kb, tmp);
+    const int4 hg = reinterpret_cast<const int4*>(d.topo.hinge)[h];
+    const double c = hinge_forces(own, hg, box, hbox, P.kb, tmp);
     he[h] = P.kb * 0.25 * (1 - fabs(c));  // src/edge.cpp:64-66 after the flip: n0.n1 = |cos|
   }
   __syncthreads();
   double e[5] = {0, 0, 0, 0, 0};
   if (active) {
     const double4 pi = own[tid];
-    for (int k = d.topo.e_off[tid]; k < d.topo.e_off[tid + 1]; k++) {  // src/bead.cpp:9-14
-      const double4 po = own[d.topo.e_other[k]];
+    const int ecnt = d.topo.e_cnt[tid];
+    for (int k = 0; k < ecnt; k++) {  // src/bead.cpp:9-14
+      const double4 po = own[d.topo.e_other[tid * kMaxDeg + k]];
       const double dx = min_img(pi.x - po.x, box, hbox), dy = min_img(pi.y - po.y, box, hbox),
                    dz = min_img(pi.z - po.z, box, hbox);
       const double L = sqrt(dx * dx + dy * dy + dz * dz);
-      const double dl = d.topo.e_len0[k] - L;
+      const double dl = d.topo.e_len0[tid * kMaxDeg + k] - L;
       e[0] += 0.25 * P.ks * dl * dl;
     }
-    for (int k = d.topo.h_off[tid]; k < d.topo.h_off[tid + 1]; k++) e[1] += he[d.topo.h_slot[k] >> 2];
+    const int hcnt = d.topo.h_cnt[tid];
+    for (int k = 0; k < hcnt; k++) e[1] += he[d.topo.h_slot[tid * kMaxHinc + k] >> 2];
     // LJ / WCA energies (src/energies.cpp:63-90): s = (s_i+s_j)/2, shifted attractive form
     const int ti = d.topo.type[tid];
     const unsigned long long mask_i = d.topo.att_mask[ti];
@@ -644,10 +1027,6 @@ __global__ void k_bd_uniforms(Dev d, uint32_t state, int nb, double* out) {
 }
 
 // ------------------------------------------------------------------------------------------
-static size_t force_smem(const Dev& d) {
-  return (size_t)(2 * d.topo.np) * sizeof(double4) + (size_t)d.topo.nh * 12 * sizeof(double);
-}
-
 static int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -669,25 +1048,28 @@ int launch_integrate(const Dev& d, int mode, hipStream_t st) {
   return check_launch();
 }
 
-int launch_force(const Dev& d, int mode, int init, int force_rebuild, int fp32, hipStream_t st) {
+int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st) {
+  hipLaunchKernelGGL(k_rebuild, dim3(d.Ns), dim3(kBlock), 0, st, d, force_rebuild);
+  return check_launch();
+}
+
+int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st) {
   dim3 g(d.Ns), b(kBlock);
-  size_t sm = force_smem(d);
+  const size_t sm = force_lds_bytes(d.topo.np, d.topo.nh, d.topo.nq);
   if (fp32) {
-    if (mode == MODE_FORCES) hipLaunchKernelGGL((k_force<MODE_FORCES, float>), g, b, sm, st, d, init, force_rebuild);
-    else if (mode == MODE_MD) hipLaunchKernelGGL((k_force<MODE_MD, float>), g, b, sm, st, d, init, force_rebuild);
-    else hipLaunchKernelGGL((k_force<MODE_BD, float>), g, b, sm, st, d, init, force_rebuild);
+    if (mode == MODE_FORCES) hipLaunchKernelGGL((k_force<MODE_FORCES, float>), g, b, sm, st, d, init);
+    else if (mode == MODE_MD) hipLaunchKernelGGL((k_force<MODE_MD, float>), g, b, sm, st, d, init);
+    else hipLaunchKernelGGL((k_force<MODE_BD, float>), g, b, sm, st, d, init);
   } else {
-    if (mode == MODE_FORCES) hipLaunchKernelGGL((k_force<MODE_FORCES, double>), g, b, sm, st, d, init, force_rebuild);
-    else if (mode == MODE_MD) hipLaunchKernelGGL((k_force<MODE_MD, double>), g, b, sm, st, d, init, force_rebuild);
-    else hipLaunchKernelGGL((k_force<MODE_BD, double>), g, b, sm, st, d, init, force_rebuild);
+    if (mode == MODE_FORCES) hipLaunchKernelGGL((k_force<MODE_FORCES, double>), g, b, sm, st, d, init);
+    else if (mode == MODE_MD) hipLaunchKernelGGL((k_force<MODE_MD, double>), g, b, sm, st, d, init);
+    else hipLaunchKernelGGL((k_force<MODE_BD, double>), g, b, sm, st, d, init);
   }
   return check_launch();
 }
 
-int launch_finalize(const Dev& d, int mode, int init, hipStream_t st) {
-  // `init` doubles as the forced-rebuild flag: a start always builds the lists
-  hipLaunchKernelGGL(k_finalize<0>, dim3(d.R), dim3(256), 0, st, d, init, init);
-  (void)mode;
+int launch_finalize(const Dev& d, int init, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(d.R), dim3(kFinThreads), 0, st, d, init);
   return check_launch();
 }
 

@@ -57,6 +57,13 @@ inline double make_tag(int s, int l) {
   return d;
 }
 
+constexpr int kMaxDeg = 8;        // edges per bead (padded rows)
+constexpr int kMaxHinc = 16;      // hinge incidences per bead (padded rows)
+constexpr int kRbRecords = 1536;  // LDS records of candidate beads at a list build
+constexpr int kRbPipe = 4;        // candidates whose beads a list build fetches together
+constexpr int kRbGrid = 8;        // local build grid: at most 8 cells per axis
+constexpr int kRbLjBuf = 24;      // per-thread LDS buffer of a bead's LJ list at a build
+
 // Device view of the template tables.
 struct TopoDev {
   int np, nh, nq, centre;
@@ -65,12 +72,14 @@ struct TopoDev {
   const int* type;
   const int* qslot;
   const int* qlocal;
-  const int* e_off;
-  const int* e_other;
-  const double* e_len0;
-  const int* hinge;   // [nh][4]
-  const int* h_off;
-  const int* h_slot;
+  // per-bead padded rows: incident edges (other bead, rest length) and hinge incidences
+  // (slot 4 h + k), so a thread fetches its whole row with independent loads
+  const int* e_cnt;               // [np]
+  const unsigned char* e_other;   // [np][kMaxDeg]
+  const double* e_len0;           // [np][kMaxDeg]
+  const int* h_cnt;               // [np]
+  const unsigned short* h_slot;   // [np][kMaxHinc]
+  const int* hinge;      // [nh][4]
   const unsigned long long* att_mask;  // [kMaxTypes]
 };
 
@@ -83,6 +92,10 @@ struct RepParams {
   double sub_rc_lj2, sub_rc_q2;  // candidate-subunit centre distance^2 for a list build
   double rc_q_list2;             // (ecut_el + skin)^2
   double r_bound2;               // largest allowed bead distance^2 from its centre bead
+  double r_bound;                // its square root
+  double r_bound_q2;             // the same bound^2 for charged beads
+  double rl_lj;                  // largest LJ/WCA list radius (cutoff + skin) over bead pairs
+  int grid_ok;                   // 1 if the local build grid fits in half a box
   double inv_nb;                 // 1.0 / beads of the replica
   double Qb[kMaxChain];          // bath masses
   double dof[kMaxChain];         // bath degrees of freedom (as double)
@@ -138,8 +151,9 @@ enum { MODE_FORCES = 0, MODE_MD = 1, MODE_BD = 2 };
 // kernel launchers (kernels.hip)
 int launch_centres(const Dev& d, hipStream_t st);
 int launch_integrate(const Dev& d, int mode, hipStream_t st);
-int launch_force(const Dev& d, int mode, int init, int force_rebuild, int fp32, hipStream_t st);
-int launch_finalize(const Dev& d, int mode, int init, hipStream_t st);
+int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st);
+int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st);
+int launch_finalize(const Dev& d, int init, hipStream_t st);
 int launch_apply_com(const Dev& d, hipStream_t st);
 int launch_energy(const Dev& d, hipStream_t st);
 int launch_bd_uniforms_state(uint32_t state, int nb, double* out_dev, const Dev& d, hipStream_t st);

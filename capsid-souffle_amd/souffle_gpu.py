@@ -36,6 +36,7 @@ class SfStats(ctypes.Structure):
                 ("q_entries", ctypes.c_int64), ("lj_capacity", ctypes.c_int32), ("q_capacity", ctypes.c_int32),
                 ("n_beads", ctypes.c_int32), ("n_subunits", ctypes.c_int32), ("n_charged", ctypes.c_int32),
                 ("n_replicas", ctypes.c_int32), ("k_integrate_ms", ctypes.c_double),
+                ("k_rebuild_ms", ctypes.c_double),
                 ("k_force_ms", ctypes.c_double), ("k_reduce_ms", ctypes.c_double),
                 ("k_timed_steps", ctypes.c_double)]
 

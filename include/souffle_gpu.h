@@ -81,7 +81,8 @@ typedef struct {
   int32_t lj_capacity;    /* current per-bead capacities */
   int32_t q_capacity;
   int32_t n_beads, n_subunits, n_charged, n_replicas;
-  double k_integrate_ms;  /* mean kernel durations over the last timed sf_step */
+  double k_integrate_ms;  /* mean kernel durations over the last timed sf_step (HIP events on the */
+  double k_rebuild_ms;    /* library's stream around every 10th step's kernels) */
   double k_force_ms;
   double k_reduce_ms;
   double k_timed_steps;   /* steps the means were taken over (0 if timing was off) */
