@@ -167,15 +167,22 @@ __device__ int find_candidates(const Dev& d, const RepParams& P, RepState& S, in
     const int per = (span + kBlock - 1) / kBlock;
     unsigned long long bl = 0, bq = 0;
     const int t0 = P.s0 + base + tid * per;
-    for (int m = 0; m < per; m++) {
-      const int t = t0 + m;
-      if (t >= P.s0 + base + span) break;
-      const double4 ct = d.centres[t];
-      const double dx = min_img(cs.x - ct.x, box, hbox), dy = min_img(cs.y - ct.y, box, hbox),
-                   dz = min_img(cs.z - ct.z, box, hbox);
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      if (t != s && d2 < P.sub_rc_lj2) bl |= 1ull << m;
-      if (nq > 0 && d2 < P.sub_rc_q2) bq |= 1ull << m;
+    const int lim = P.s0 + base + span;
+    for (int m0 = 0; m0 < per; m0 += 8) {
+      double4 ct[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++)  // eight centres requested together
+        if (m0 + u < per && t0 + m0 + u < lim) ct[u] = d.centres[t0 + m0 + u];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int m = m0 + u, t = t0 + m;
+        if (m >= per || t >= lim) break;
+        const double dx = min_img(cs.x - ct[u].x, box, hbox), dy = min_img(cs.y - ct[u].y, box, hbox),
+                     dz = min_img(cs.z - ct[u].z, box, hbox);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (t != s && d2 < P.sub_rc_lj2) bl |= 1ull << m;
+        if (nq > 0 && d2 < P.sub_rc_q2) bq |= 1ull << m;
+      }
     }
     const int cnt = __popcll(bl | bq);
     scan[tid] = cnt;
@@ -292,7 +299,8 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
   __shared__ float4 rec[kRbRecords];
   __shared__ int cell_cnt[kRbGrid * kRbGrid * kRbGrid];
   __shared__ int cell_off[kRbGrid * kRbGrid * kRbGrid + 1];
-  __shared__ int ljbuf[kRbLjBuf * kBlock];
+  __shared__ int ljbuf[kRbLjBuf * kBlock];  // also the Yukawa record masks (after the LJ build)
+  __shared__ float4 qown[32];
   __shared__ int fallback_sh;
   __shared__ float tsig[kBlock];
   __shared__ int ttype[kBlock];
@@ -456,8 +464,13 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
     __syncthreads();
     nqc = cell_off[0];
     const int nrec = nqc * nq;
-    q_fast = nrec <= kRbRecords && nqc <= kRbGrid * kRbGrid * kRbGrid;
+    q_fast = nrec <= kRbRecords && nqc <= kRbGrid * kRbGrid * kRbGrid && nq <= 32;
     if (q_fast) {
+      // stage the candidates' charged beads in candidate order, and this subunit's own
+      const int qi = active ? d.topo.qslot[tid] : -1;
+      if (qi >= 0)
+        qown[qi] = make_float4((float)min_img(pi.x - cs.x, box, hbox), (float)min_img(pi.y - cs.y, box, hbox),
+                               (float)min_img(pi.z - cs.z, box, hbox), __int_as_float(gi));
       for (int f = tid; f < nrec; f += kBlock) {
         const int c = f / nq, kq = f - c * nq;
         const int j = cell_cnt[c] * np + d.topo.qlocal[kq];
@@ -466,21 +479,29 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
                              (float)min_img(pj.z - cs.z, box, hbox), __int_as_float(j));
       }
       __syncthreads();
-      const int qi = active ? d.topo.qslot[tid] : -1;
+      // every thread tests its records against all charged beads of s: bit kq of ymask[f] says
+      // record f is in the list of charged bead kq
+      unsigned* ymask = reinterpret_cast<unsigned*>(ljbuf);
+      const float rq = sqrtf((float)P.rc_q_list2) + 1e-3f;
+      const float rq2 = rq * rq;
+      for (int f = tid; f < nrec; f += kBlock) {
+        const float4 rj = rec[f];
+        const int j = __float_as_int(rj.w);
+        unsigned msk = 0;
+        for (int kq = 0; kq < nq; kq++) {
+          const float4 oi = qown[kq];
+          const float dx = min_imgf(oi.x - rj.x, boxf, hboxf), dy = min_imgf(oi.y - rj.y, boxf, hboxf),
+                      dz = min_imgf(oi.z - rj.z, boxf, hboxf);
+          if (j != __float_as_int(oi.w) && dx * dx + dy * dy + dz * dz < rq2) msk |= 1u << kq;
+        }
+        ymask[f] = msk;
+      }
+      __syncthreads();
       if (qi >= 0) {
         const int gq = s * nq + qi;
-        const float rq = sqrtf((float)P.rc_q_list2) + 1e-3f;
-        const float rq2 = rq * rq;
-        const float ix = (float)min_img(pi.x - cs.x, box, hbox), iy = (float)min_img(pi.y - cs.y, box, hbox),
-                    iz = (float)min_img(pi.z - cs.z, box, hbox);
         for (int f = 0; f < nrec; f++) {
-          const float4 rj = rec[f];
-          const int j = __float_as_int(rj.w);
-          if (j == gi) continue;
-          const float dx = min_imgf(ix - rj.x, boxf, hboxf), dy = min_imgf(iy - rj.y, boxf, hboxf),
-                      dz = min_imgf(iz - rj.z, boxf, hboxf);
-          if (dx * dx + dy * dy + dz * dz < rq2) {
-            if (cnt_q < d.q_cap) d.q_list[(size_t)cnt_q * d.Nq + gq] = j;
+          if ((ymask[f] >> qi) & 1u) {
+            if (cnt_q < d.q_cap) d.q_list[(size_t)cnt_q * d.Nq + gq] = __float_as_int(rec[f].w);
             cnt_q++;
           }
         }
@@ -649,6 +670,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
   const int4 hrow0 = reinterpret_cast<const int4*>(d.topo.h_slot)[2 * lt];
   const int4 hrow1 = reinterpret_cast<const int4*>(d.topo.h_slot)[2 * lt + 1];
   const double m = d.topo.mass[lt];
+  const int4 hg0 = reinterpret_cast<const int4*>(d.topo.hinge)[tid < nh ? tid : 0];
   double vx = 0, vy = 0, vz = 0;
   if (active) { vx = d.vx[gi]; vy = d.vy[gi]; vz = d.vz[gi]; }
   // offsets of each bead's entries in the subunit's concatenated lists (LJ entries, then
@@ -671,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
   if (tid == 0) off_s[kBlock] = total;
   // bending hinges -> LDS
   for (int h = tid; h < nh; h += kBlock) {
-    const int4 hg = reinterpret_cast<const int4*>(d.topo.hinge)[h];
+    const int4 hg = (h == tid) ? hg0 : reinterpret_cast<const int4*>(d.topo.hinge)[h];
     hinge_forces(own, hg, box, hbox, kb, hc + 12 * h);
   }
   // stretching (src/bead.cpp:17-23) over the bead's edge row

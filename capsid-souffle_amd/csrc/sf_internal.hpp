@@ -60,7 +60,7 @@ inline double make_tag(int s, int l) {
 constexpr int kMaxDeg = 8;        // edges per bead (padded rows)
 constexpr int kMaxHinc = 16;      // hinge incidences per bead (padded rows)
 constexpr int kRbRecords = 1536;  // LDS records of candidate beads at a list build
-constexpr int kRbPipe = 4;        // candidates whose beads a list build fetches together
+constexpr int kRbPipe = 8;        // candidates whose beads a list build fetches together
 constexpr int kRbGrid = 8;        // local build grid: at most 8 cells per axis
 constexpr int kRbLjBuf = 24;      // per-thread LDS buffer of a bead's LJ list at a build
 
