@@ -5,8 +5,10 @@ C = 800 uM, c = 100 mM, E = 2.1, dt = 0.005, K = 298, ks = 50, kb = 20, Nose-Hoo
 q = 5, Q = 10, Yukawa cutoff 12/kappa, Verlet skin 0.3 sigma; seeded synthetic initial state
 (non-overlapping random subunit positions and orientations, Maxwell-Boltzmann velocities).
 A step is one iteration of the reference's step loop body (src/md.cpp:400-514) over all beads.
-The metric is bead-steps/s = beads x timed steps / wall seconds of the timed steps, inputs
-resident in HBM. Prints one JSON line.
+The metric is bead-steps/s = beads x timed steps / wall seconds of the timed steps, with the
+state resident in HBM; steps run as HIP-graph chunks. A second pass of the same length runs
+with HIP events around every 10th step's kernels (on the library's stream) for the per-kernel
+durations the roofline uses. Prints one JSON line.
 """
 import argparse
 import json
@@ -18,44 +20,70 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "capsid-souffle_amd"))
 sys.path.insert(0, os.path.join(REPO, "tests"))
 
-HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector
+CFG = dict(C_uM=800.0, c_mM=100.0, elj_att=2.1, dt=0.005, K=298.0, q_chain=5, Q=10.0, damp=100.0)
 
 
-def force_kernel_bytes_per_bead(n_list, k_rebuild):
-    """Algorithmic HBM bytes per bead of one k_force launch: own position 24, velocity read
-    and write 48, force write 24, per-bead parameters 8, 4 bytes per list entry read, and the
-    amortised list build (24 + 4 n_list) / k_rebuild. Neighbour positions are counted once per
-    bead (as the bead's own position read) — re-reads are what `traffic` exposes."""
-    return 104.0 + 4.0 * n_list + (24.0 + 4.0 * n_list) / max(k_rebuild, 1.0)
+def force_kernel_bytes_per_bead(n_list):
+    """Algorithmic HBM bytes per bead of one k_force launch: own position 24, velocity read and
+    write 48, force write 24, per-bead parameters 8, and 4 bytes per list entry. Neighbour
+    positions are not counted (ideally served from cache); `traffic` measures what is fetched."""
+    return 104.0 + 4.0 * n_list
 
 
-def run_gpu(args, brownian, steps, warmup, fp32=False):
+def step_bytes_per_bead(n_list, k_rebuild):
+    """Whole-step algorithmic bytes per bead-step: x r/w 48, v r/w 48, F w+r 48, parameters 8,
+    list entries 4 n_list, list build (24 + 4 n_list) / k_rebuild."""
+    return 152.0 + 4.0 * n_list + (24.0 + 4.0 * n_list) / max(k_rebuild, 1.0)
+
+
+def measured_traffic(args):
+    """HBM bytes per k_force launch from the committed rocprofv3 PMC summary, when it was
+    taken on this workload; None otherwise."""
+    path = os.path.join(REPO, "profiles", "k_force_trimer2000_nh_fp64.traffic.txt")
+    if args.subunits != 2000 or abs(args.skin - 0.3) > 1e-12 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("k_force_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def make_system(args, brownian=False, fp32=False, skin=None):
     import souffle_gpu as sg
     from oracle_lib import TEMPLATE
     t = sg.Template(TEMPLATE)
-    p = t.params(C_uM=800.0, c_mM=100.0, elj_att=2.1, dt=0.005, K=298.0, n_subunits=args.subunits,
-                 brownian=1 if brownian else 0, q_chain=5, Q=10.0, damp=100.0)
-    p.skin = args.skin
+    p = t.params(n_subunits=args.subunits, brownian=1 if brownian else 0, **CFG)
+    p.skin = args.skin if skin is None else skin
     p.pair_fp32 = 1 if fp32 else 0
     pos, vel = t.make_state(args.subunits, p.box, p.T, 2000)
     s = sg.System(t, args.subunits, p)
     s.set_state(pos, vel)
-    s.step(warmup)
+    return s, pos, vel
+
+
+def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
+    s, pos, vel = make_system(args, brownian, fp32, skin)
+    s.step(args.warmup)
     st0 = s.stats()
-    s.set_timing(True)
     t0 = time.perf_counter()
-    s.step(steps)
+    s.step(args.steps)
     t1 = time.perf_counter()
     st1 = s.stats()
-    s.set_timing(False)
     wall = t1 - t0
     nb = s.n_beads
-    rebuilds = st1["rebuilds"] - st0["rebuilds"]
-    n_list = (st1["lj_entries"] + st1["q_entries"]) / nb
-    k_rebuild = steps / max(rebuilds, 1)
-    return dict(nb=nb, wall=wall, value=nb * steps / wall, ms_per_step=1e3 * wall / steps,
-                n_list=n_list, k_rebuild=k_rebuild, rebuilds=rebuilds, stats=st1)
+    out = dict(nb=nb, wall=wall, value=nb * args.steps / wall, ms_per_step=1e3 * wall / args.steps,
+               rebuilds=st1["rebuilds"] - st0["rebuilds"],
+               n_list=(st1["lj_entries"] + st1["q_entries"]) / nb)
+    out["k_rebuild"] = args.steps / max(out["rebuilds"], 1)
+    if timed_pass:
+        s.set_timing(True)
+        s.step(args.steps)
+        s.set_timing(False)
+        out["stats"] = s.stats()
+    out["system"] = s
+    return out
 
 
 def run_cpu(args):
@@ -65,14 +93,27 @@ def run_cpu(args):
     import souffle_gpu as sg
     from oracle_lib import TEMPLATE, Oracle
     t = sg.Template(TEMPLATE)
-    p = t.params(C_uM=800.0, c_mM=100.0, elj_att=2.1, dt=0.005, K=298.0, n_subunits=args.subunits)
+    p = t.params(n_subunits=args.subunits, **CFG)
     pos, vel = t.make_state(args.subunits, p.box, p.T, 2000)
-    threads = min(os.cpu_count() or 1, args.cpu_threads) if args.cpu_threads > 0 else (os.cpu_count() or 1)
-    o = Oracle(args.subunits, C_uM=800.0, c_mM=100.0, elj_att=2.1, dt=0.005, K=298.0, L=0.3, B=5)
+    threads = os.cpu_count() or 1
+    if args.cpu_threads > 0:
+        threads = min(threads, args.cpu_threads)
+    o = Oracle(args.subunits, C_uM=CFG["C_uM"], c_mM=CFG["c_mM"], elj_att=CFG["elj_att"], dt=CFG["dt"],
+               K=CFG["K"], Q=CFG["Q"], q_chain=CFG["q_chain"], L=0.3, B=5)
     o.set_state(pos, vel)
     o.time_steps(1, threads)
     secs = o.time_steps(args.cpu_steps, threads)
     return dict(value=o.N * args.cpu_steps / secs, threads=threads, secs=secs)
+
+
+def probe_copy(mb):
+    """A plain device-to-device copy of known size (for scaling FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+    n = mb * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
 
 
 def main():
@@ -82,25 +123,29 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--subunits", type=int, default=2000)
     ap.add_argument("--skin", type=float, default=0.3)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BD and FP32-pair runs")
+    ap.add_argument("--probe-copy-mb", type=int, default=0, help="run a known-size copy kernel first")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the system would shard by "
-              "subunit-owning spatial domains with a halo exchange; see the design notes).", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (one large system would shard "
+              "into subunit-owning spatial domains with one halo exchange per step).", file=sys.stderr)
         sys.exit(2)
     import souffle_gpu as sg
     if not os.path.exists(sg.LIB_PATH):
         import __graft_entry__
         __graft_entry__.build()
+    if args.probe_copy_mb > 0:
+        probe_copy(args.probe_copy_mb)
 
-    md = run_gpu(args, brownian=False, steps=args.steps, warmup=args.warmup)
+    md = run_gpu(args)
     st = md["stats"]
     t_force = st["k_force_ms"]
-    b_force = force_kernel_bytes_per_bead(md["n_list"], md["k_rebuild"])
+    b_force = force_kernel_bytes_per_bead(md["n_list"])
     achieved = b_force * md["nb"] / (t_force * 1e-3) / 1e9 if t_force > 0 else 0.0
+    b_step = step_bytes_per_bead(md["n_list"], md["k_rebuild"])
     out = {
         "metric": "bead-steps/s",
         "value": md["value"],
@@ -114,26 +159,42 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"trimer_MVM x{args.subunits} plain FP64 Nose-Hoover MD (q=5), "
-                               f"C=800uM c=100mM E=2.1 dt=0.005 K=298, skin {args.skin}",
+        "config": {"workload": f"trimer_MVM x{args.subunits} plain FP64 Nose-Hoover MD (q=5, Q=10), "
+                               f"C=800uM c=100mM E=2.1 dt=0.005 K=298 ks=50 kb=20, skin {args.skin}",
                    "subunits": args.subunits, "beads": md["nb"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-scaled)",
+                     "algorithmic_bytes_per_launch": b_force * md["nb"],
                      "kernel": "k_force", "kernel_ms": t_force, "bytes_per_bead": b_force,
-                     "n_list": md["n_list"], "k_rebuild": md["k_rebuild"]},
-        "kernels_ms": {"k_integrate": st["k_integrate_ms"], "k_force": t_force, "k_finalize": st["k_reduce_ms"]},
+                     "n_list": md["n_list"], "k_rebuild": md["k_rebuild"],
+                     "step_bytes_per_bead": b_step,
+                     "step_achieved_GBs": b_step * md["value"] / 1e9},
+        "kernels_ms": {"k_integrate": st["k_integrate_ms"], "k_rebuild": st["k_rebuild_ms"],
+                       "k_force": t_force, "k_finalize": st["k_reduce_ms"]},
     }
     if not args.no_extra:
-        bd = run_gpu(args, brownian=True, steps=args.steps, warmup=args.warmup)
-        f32 = run_gpu(args, brownian=False, steps=args.steps, warmup=args.warmup, fp32=True)
+        import numpy as np
+        bd = run_gpu(args, brownian=True, timed_pass=False)
+        f32 = run_gpu(args, fp32=True, timed_pass=False)
+        # FP32-pair force error against FP64 on the same state (after the FP64 run's steps)
+        pos, vel = md["system"].get_state()
+        f32["system"].set_state(pos, vel)
+        md["system"].set_state(pos, vel)
+        f64 = md["system"].forces()[0]
+        ff = f32["system"].forces()[0]
+        rms = float(np.sqrt(np.sum((ff - f64) ** 2) / np.sum(f64 ** 2)))
         out["extra"] = {"bd_bead_steps_per_s": bd["value"], "bd_ms_per_step": bd["ms_per_step"],
-                        "fp32_pair_bead_steps_per_s": f32["value"], "fp32_pair_ms_per_step": f32["ms_per_step"]}
+                        "fp32_pair_bead_steps_per_s": f32["value"], "fp32_pair_ms_per_step": f32["ms_per_step"],
+                        "fp32_pair_force_rel_rms_err": rms}
+    for k in ("system",):
+        md.pop(k, None)
     if not args.no_cpu:
         cpu = run_cpu(args)
         out["cpu_baseline"] = {"value": cpu["value"], "unit": "bead-steps/s", "cores": cpu["threads"],
                                "kind": "port",
                                "sample": f"{args.cpu_steps} NH steps of the same {args.subunits}-subunit state "
-                                         f"(oracle, FP64, reference list algorithm at -L 0.3 -B 5), "
+                                         f"(oracle, FP64, the reference list algorithm at -L 0.3 -B 5), "
                                          f"{cpu['secs']:.1f} s"}
     print(json.dumps(out), flush=True)
 
