@@ -612,6 +612,7 @@ int sf_get_stats(sf_handle* h, sf_stats* o) {
   o->n_beads = h->Nb; o->n_subunits = h->Ns; o->n_charged = h->Nq; o->n_replicas = h->R;
   o->k_integrate_ms = h->t_int; o->k_rebuild_ms = h->t_rb; o->k_force_ms = h->t_force; o->k_reduce_ms = h->t_red;
   o->k_timed_steps = h->t_steps;
+  o->vcom = s[0].vcom;
   return SF_OK;
 }
 

@@ -1,0 +1,380 @@
+// souffle-gpu: command-line driver over the C ABI, mirroring the reference's run_simulation
+// (src/md.cpp:29-693): the same flags and defaults (src/md.cpp:53-90), the template from
+// infiles/<-f>, resume from the newest outfiles/restart_*.out (src/initialize.cpp:166-191), and
+// the outfiles/ writers: energy.out (src/initialize.cpp:12-19, src/md.cpp:339-346, 616-623),
+// model.parameters.out (src/md.cpp:112-180, 234), restart_<a>.out (src/md.cpp:526-536),
+// ovito.lammpstrj (src/md.cpp:647-669) and the progress bar (src/functions.cpp:24-30).
+// The step loop runs on the GPU in blocks between output steps.
+//
+// Initial state without a restart file: the reference's cubic lattice of identically oriented
+// subunits (src/initialize.cpp:199-212) and its per-subunit uniform velocities
+// (src/initialize.cpp:471-545). The reference draws those from GSL's default generator
+// (mt19937) seeded with the first random() after srand(0) (src/rand_gen.h:18-25); std::mt19937
+// with the same seed yields the same 32-bit stream and gsl_rng_uniform is x / 2^32.
+#include <dirent.h>
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <random>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "souffle_gpu.h"
+
+namespace {
+
+struct Opts {
+  char response = 'm';
+  std::string file = "41part_c";
+  double C = 200, c = 500, S = 64, ks = 50, kb = 20, T = 2500, dt = 0.004, r = 1, damp = 100, q = 5, Q = 10,
+         K = 298, e = 12, E = 2, L = 7.5, a = 3.0, G = -78, H = 3.5;
+  bool R = true, X = false, V = true, p = false;
+  int B = 20, M = 1000, W = 1000, w = 1000000, N = 4, m = 1;
+  double skin = 0.3;       // GPU Verlet skin (no reference counterpart; -L is the reference's list)
+  int fp32 = 0;            // --fp32 1: FP32 pair arithmetic
+};
+
+bool parse_bool(const std::string& s) { return s == "1" || s == "true" || s == "yes" || s == "on"; }
+
+int parse(int argc, char** argv, Opts& o) {
+  std::map<std::string, std::string> kv;
+  for (int i = 1; i < argc; i++) {
+    std::string a = argv[i];
+    if (a == "-h" || a == "--help") {
+      std::cout << "Usage:\nsouffle-gpu <options>  (flags of the reference: -D -f -C -c -S -s -b -T -t -r -d -q -Q -K -e\n"
+                   "  -R -X -V -E -B -L -M -W -w -N -p -a -G -m -H; GPU only: --skin <sigma>, --fp32 <0|1>)\n";
+      return 1;
+    }
+    std::string key, val;
+    if (a.rfind("--", 0) == 0) {
+      key = a.substr(2);
+      if (i + 1 >= argc) { std::cerr << "missing value for " << a << "\n"; return -1; }
+      val = argv[++i];
+    } else if (a.size() >= 2 && a[0] == '-') {
+      key = a.substr(1, 1);
+      if (a.size() > 2) val = a.substr(2);
+      else {
+        if (i + 1 >= argc) { std::cerr << "missing value for " << a << "\n"; return -1; }
+        val = argv[++i];
+      }
+    } else {
+      std::cerr << "unexpected argument " << a << "\n";
+      return -1;
+    }
+    kv[key] = val;
+  }
+  for (auto& it : kv) {
+    const std::string& k = it.first;
+    const std::string& v = it.second;
+    if (k == "D") o.response = v.empty() ? 'm' : v[0];
+    else if (k == "f") o.file = v;
+    else if (k == "C") o.C = atof(v.c_str());
+    else if (k == "c") o.c = atof(v.c_str());
+    else if (k == "S") o.S = atof(v.c_str());
+    else if (k == "s") o.ks = atof(v.c_str());
+    else if (k == "b") o.kb = atof(v.c_str());
+    else if (k == "T") o.T = atof(v.c_str());
+    else if (k == "t") o.dt = atof(v.c_str());
+    else if (k == "r") o.r = atof(v.c_str());
+    else if (k == "d") o.damp = atof(v.c_str());
+    else if (k == "q") o.q = atof(v.c_str());
+    else if (k == "Q") o.Q = atof(v.c_str());
+    else if (k == "K") o.K = atof(v.c_str());
+    else if (k == "e") o.e = atof(v.c_str());
+    else if (k == "R") o.R = parse_bool(v);
+    else if (k == "X") o.X = parse_bool(v);
+    else if (k == "V") o.V = parse_bool(v);
+    else if (k == "E") o.E = atof(v.c_str());
+    else if (k == "B") o.B = atoi(v.c_str());
+    else if (k == "L") o.L = atof(v.c_str());
+    else if (k == "M") o.M = atoi(v.c_str());
+    else if (k == "W") o.W = atoi(v.c_str());
+    else if (k == "w") o.w = atoi(v.c_str());
+    else if (k == "N") o.N = atoi(v.c_str());
+    else if (k == "p") o.p = parse_bool(v);
+    else if (k == "a") o.a = atof(v.c_str());
+    else if (k == "G") o.G = atof(v.c_str());
+    else if (k == "m") o.m = atoi(v.c_str());
+    else if (k == "H") o.H = atof(v.c_str());
+    else if (k == "skin") o.skin = atof(v.c_str());
+    else if (k == "fp32") o.fp32 = atoi(v.c_str());
+    else { std::cerr << "unknown option -" << k << "\n"; return -1; }
+  }
+  return 0;
+}
+
+// The newest restart file in outfiles/ (names containing "restart", ordered by length then
+// lexicographically, src/functions.cpp:262-307).
+std::string newest_restart() {
+  std::vector<std::string> names;
+  if (DIR* d = opendir("outfiles")) {
+    while (dirent* e = readdir(d)) {
+      std::string n = e->d_name;
+      if (n.find("restart") != std::string::npos) names.push_back(n);
+    }
+    closedir(d);
+  }
+  if (names.empty()) return "";
+  std::sort(names.begin(), names.end(), [](const std::string& a, const std::string& b) {
+    if (a.size() != b.size()) return a.size() < b.size();
+    return a < b;
+  });
+  return names.back();
+}
+
+struct Template {
+  std::vector<double> x, y, z, mass, q;
+  std::vector<int> type;
+  double si_sigma = 0, si_mass = 0, bondlength = 0;
+};
+
+bool read_template_beads(const std::string& path, Template& t) {
+  std::ifstream in(path.c_str());
+  if (!in) return false;
+  std::string d;
+  int np;
+  in >> d >> d >> t.bondlength >> d >> d >> t.si_mass >> d >> d >> t.si_sigma;
+  in >> d >> d >> d >> d >> d >> np;
+  for (int i = 0; i < 8; i++) in >> d;
+  for (int i = 0; i < np; i++) {
+    long long name, type;
+    double x, y, z, c, r, m;
+    in >> name >> x >> y >> z >> type >> c >> r >> m;
+    t.x.push_back(x); t.y.push_back(y); t.z.push_back(z);
+    t.type.push_back((int)type); t.q.push_back(c); t.mass.push_back(m);
+  }
+  return (bool)in;
+}
+
+void progress(double fraction) {  // src/functions.cpp:24-30
+  static const char* bar = "||||||||||||||||||||||||||||||||||||||||||||||||||||||||||||";
+  int val = (int)(fraction * 100);
+  int lpad = (int)(fraction * 60);
+  int rpad = 60 - lpad;
+  printf("\r%3d%% |%.*s%*s|", val, lpad, bar, rpad, "");
+  fflush(stdout);
+}
+
+int fail(const char* what) {
+  std::cerr << what << ": " << sf_last_error() << "\n";
+  return 1;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Opts o;
+  int pr = parse(argc, argv, o);
+  if (pr != 0) return pr < 0 ? 1 : 0;
+  if (o.p) { std::cerr << "encapsulation big beads (-p) are not part of this engine\n"; return 1; }
+  if (o.X) { std::cerr << "cluster start (-X) needs infiles/xyz_T4, which this engine does not read\n"; return 1; }
+  if (o.e == 0) { std::cerr << "uncut Yukawa (-e 0) is not supported by the list engine\n"; return 1; }
+  mkdir("outfiles", 0755);
+  const bool brownian = o.response == 'b';
+  std::ofstream traj("outfiles/energy.out", std::ios_base::app);
+  std::ofstream ofile("outfiles/ovito.lammpstrj", std::ios_base::app);
+  std::ofstream sysdata("outfiles/model.parameters.out", std::ios::out);
+  traj << "time" << std::setw(15) << "kinetic" << std::setw(15) << "stretching" << std::setw(15) << "bending"
+       << std::setw(15) << "LJ" << std::setw(15) << "electrostatics" << std::setw(15) << "total" << std::setw(15)
+       << "potential" << std::setw(15) << "temperature" << std::setw(20) << "thermo_potential" << std::setw(15)
+       << "thermo_kinetic" << std::setw(15) << "system_velocity_magnitude" << std::endl;
+  if (brownian) sysdata << "Running brownian dynamics." << std::endl;
+  else if (o.response == 'm') sysdata << "Running molecular dynamics with Nose Hoover thermostat." << std::endl;
+
+  const std::string tpath = "infiles/" + o.file;
+  sf_template* tmpl = sf_template_load(tpath.c_str());
+  if (!tmpl) {
+    std::cerr << "ERR: FILE " << o.file << " NOT OPENED. Check directory and/or filename.";
+    return 1;
+  }
+  Template tb;
+  read_template_beads(tpath, tb);
+  int np = 0;
+  sf_template_info(tmpl, &np, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  const int ns = (int)o.S;
+  const long long N = (long long)ns * np;
+
+  sf_cli cli{};
+  cli.C_uM = o.C; cli.c_mM = o.c; cli.ks = o.ks; cli.kb = o.kb; cli.elj_att = o.E; cli.ecut_c = o.e;
+  cli.dt = o.dt; cli.K = o.K; cli.Q = o.Q; cli.damp = o.damp; cli.n_subunits = ns;
+  cli.q_chain = (int)o.q; cli.brownian = brownian ? 1 : 0;
+  sf_params prm;
+  if (sf_params_derive(tmpl, &cli, &prm)) return fail("parameters");
+  prm.skin = o.skin;
+  prm.pair_fp32 = o.fp32;
+
+  // model.parameters.out (src/md.cpp:135-180)
+  const double Boltzmann = 1.3806e-23, Avagadro = 6.022e23;
+  const double SIenergy = o.K * Boltzmann;
+  const double SItime = std::sqrt(tb.si_sigma * tb.si_sigma * tb.si_mass / SIenergy);
+  const double totaltime = o.T * o.dt;
+  const double UnitEnergy = SIenergy * SItime * SItime / (tb.si_mass * tb.si_sigma * (1e-9));
+  sysdata << "Simulation will run for " << totaltime * SItime / (1e-9) << " nanoseconds with a "
+          << o.dt * SItime / (1e-12) << " picosecond timestep." << std::endl;
+  sysdata << "Capsomere concentration: " << o.C << " micromolar" << std::endl;
+  sysdata << "Salt concentration: " << o.c << " millimolar" << std::endl;
+  sysdata << "Stretching constant: " << o.ks << " KbT" << "( " << (o.ks * tb.si_mass / (SItime * SItime)) << " N/m)" << std::endl;
+  sysdata << "Bending constant: " << o.kb << " KbT" << "( " << (o.kb * tb.si_mass * tb.si_sigma * tb.si_sigma / (SItime * SItime))
+          << " J)" << std::endl;
+  sysdata << "Bondlength between beads is " << tb.bondlength << " LJ reduced units, which is "
+          << tb.bondlength * tb.si_sigma / (1e-9) << " nanometers." << std::endl;
+  sysdata << "Mass of a bead is " << tb.si_mass << " kg." << std::endl;
+  sysdata << "Diameter of a bead is " << tb.si_sigma / (1e-9) << " nanometers." << std::endl;
+  sysdata << "Total number of subunits is " << o.S << std::endl;
+  sysdata << "Temperature is " << o.K << " K" << " Which is " << prm.T << " in reduced units" << std::endl;
+  sysdata << "Attractive LJ paramerter is " << o.E << " Which is " << o.E * (SIenergy * Avagadro / (1000 * UnitEnergy))
+          << " kJ/mol." << std::endl;
+  sysdata << "Box length is " << prm.box * tb.si_sigma / (1e-9) << " nanometers." << std::endl;
+  sysdata << "Screening length is " << 1 / prm.kappa << " nanometers." << std::endl;
+  sysdata << "electrostatic cutoff is " << prm.ecut_el * tb.si_sigma / (1e-9) << " nanometers." << std::endl;
+  double nlc = (prm.ecut_el > prm.ecut ? prm.ecut_el : prm.ecut) + o.L;
+  sysdata << "Neighborlist cutoff is " << nlc * tb.si_sigma / (1e-9) << " nanometers." << std::endl;
+  sysdata << "GPU Verlet skin is " << o.skin << " sigma (lists rebuilt on the device when a bead moves more than skin/2)."
+          << std::endl;
+
+  // initial state: restart file or the reference's lattice and velocities
+  std::vector<double> pos(3 * N), vel(3 * N);
+  long long loopStart = 0;
+  bool restart = o.R;
+  std::string rname = restart ? newest_restart() : "";
+  if (rname.empty()) {
+    if (restart) std::cout << "No restart files found! Starting simulation from scratch." << std::endl;
+    restart = false;
+  }
+  if (restart) {
+    std::cout << "Restarting from: " << rname << std::endl;
+    std::ifstream rin(("outfiles/" + rname).c_str());
+    if (!rin) { std::cerr << "ERR: RESTART FILE outfiles/restart.out NOT OPENED. Check directory and/or filename."; return 1; }
+    std::string d;
+    rin >> d >> d >> d >> d >> loopStart;
+    for (long long i = 0; i < N; i++) rin >> d >> vel[3 * i] >> vel[3 * i + 1] >> vel[3 * i + 2] >> pos[3 * i] >> pos[3 * i + 1] >> pos[3 * i + 2];
+    if (!rin) { std::cerr << "restart file has fewer beads than the system\n"; return 1; }
+  } else {
+    const long double bx = prm.box;
+    const int nf = (int)std::ceil(std::pow((double)ns, 1.0 / 3.0));
+    long long idx = 0;
+    for (int i = 0; i < nf; i++)
+      for (int j = 0; j < nf; j++)
+        for (int k = 0; k < nf; k++) {
+          if (idx >= ns) continue;
+          for (int l = 0; l < np; l++) {
+            long long g = idx * np + l;
+            pos[3 * g] = (double)(((double)i * bx * (1 / (double)nf)) + (long double)tb.x[l]);
+            pos[3 * g + 1] = (double)(((double)j * bx * (1 / (double)nf)) + (long double)tb.y[l]);
+            pos[3 * g + 2] = (double)(((double)k * bx * (1 / (double)nf)) + (long double)tb.z[l]);
+          }
+          idx++;
+        }
+    srand(0);
+    std::mt19937 gen((unsigned long)random());
+    for (int s = 0; s < ns; s++) {
+      double u[3];
+      for (int a = 0; a < 3; a++) {
+        double rn = gen() / 4294967296.0;
+        u[a] = 0.5 * (rn) + (-0.5) * (1 - rn);
+      }
+      for (int l = 0; l < np; l++)
+        for (int a = 0; a < 3; a++) vel[3 * ((long long)s * np + l) + a] = u[a];
+    }
+    long double av[3] = {0, 0, 0};
+    for (long long i = 0; i < N; i++)
+      for (int a = 0; a < 3; a++) av[a] += vel[3 * i + a];
+    for (int a = 0; a < 3; a++) av[a] = av[a] * (1.0 / N);
+    for (long long i = 0; i < N; i++)
+      for (int a = 0; a < 3; a++) vel[3 * i + a] = (double)(vel[3 * i + a] - av[a]);
+    double ke = 0;
+    for (long long i = 0; i < N; i++) {
+      long double v = std::sqrt((long double)vel[3 * i] * vel[3 * i] + (long double)vel[3 * i + 1] * vel[3 * i + 1] +
+                                (long double)vel[3 * i + 2] * vel[3 * i + 2]);
+      ke += (double)(0.5 * tb.mass[i % np] * v * v);
+    }
+    const double sc = std::sqrt(prm.T / (ke / (1.5 * N)));
+    for (long long i = 0; i < 3 * N; i++) vel[i] = vel[i] * sc;
+    std::ofstream lv("outfiles/list_initial_velocities.out", std::ios::out);
+    for (int s = 0; s < ns; s++) {
+      long long g = (long long)s * np;
+      double vm = std::sqrt(vel[3 * g] * vel[3 * g] + vel[3 * g + 1] * vel[3 * g + 1] + vel[3 * g + 2] * vel[3 * g + 2]);
+      lv << g << std::setw(15) << vel[3 * g] << "  " << vel[3 * g + 1] << "  " << vel[3 * g + 2] << "  " << vm << std::endl;
+    }
+  }
+
+  sf_handle* h = sf_create(tmpl, 1, &ns, &prm);
+  if (!h) return fail("sf_create");
+  if (sf_set_state(h, pos.data(), vel.data())) return fail("sf_set_state");
+  std::vector<double> energies(7), per_bead(5 * N);
+  double vcom = 0;
+  auto energy_row = [&](long long a, bool with_vcom) -> int {
+    if (sf_energies(h, energies.data(), per_bead.data())) return fail("sf_energies");
+    const double ke = energies[0], se = energies[1], be = energies[2], ne = energies[3], ce = energies[4];
+    const double tp = energies[5], tk = energies[6];
+    const double tot = se + ke + ne + be + ce + tp + tk;
+    traj << a << std::setw(15) << ke / N << std::setw(15) << se / N << std::setw(15) << be / N << std::setw(15) << ne / N
+         << std::setw(15) << ce / N << std::setw(15) << tot / N << std::setw(15) << (be + se + ne + ce) / N << std::setw(15)
+         << ke * 2 / (3 * N) << std::setw(15) << tp / N << std::setw(15) << tk / N;
+    if (with_vcom) traj << std::setw(15) << vcom;
+    traj << std::endl;
+    return 0;
+  };
+  if (energy_row(0, false)) return 1;
+
+  const double nsteps_d = totaltime / o.dt;
+  long long a = loopStart;
+  double percentagePre = -1;
+  std::ofstream restart_out;
+  while ((double)a < nsteps_d + 1) {
+    // run up to the next iteration that writes output (or the last one)
+    long long stop = a;
+    while ((double)(stop + 1) < nsteps_d + 1 && stop % o.W != 0 && stop % o.w != 0 && stop % o.M != 0) stop++;
+    if (sf_step(h, (int)(stop - a + 1))) return fail("sf_step");
+    a = stop;
+    if (a % o.w == 0) {  // src/md.cpp:526-536
+      sf_get_state(h, pos.data(), vel.data());
+      std::ostringstream nm;
+      nm << "outfiles/restart_" << a << ".out";
+      restart_out.open(nm.str().c_str(), std::ios::out);
+      restart_out << "Velocities & Positions for " << a << std::endl;
+      for (long long i = 0; i < N; i++)
+        restart_out << i << "  " << vel[3 * i] << std::setw(25) << std::setprecision(12) << vel[3 * i + 1] << std::setw(25)
+                    << std::setprecision(12) << vel[3 * i + 2] << std::setw(25) << std::setprecision(12) << pos[3 * i]
+                    << std::setw(25) << std::setprecision(12) << pos[3 * i + 1] << std::setw(25) << std::setprecision(12)
+                    << pos[3 * i + 2] << std::setw(25) << std::setprecision(12) << std::endl;
+      restart_out.close();
+    }
+    if (a % o.W == 0) {  // src/md.cpp:551-623
+      sf_stats st;
+      if (sf_get_stats(h, &st)) return fail("sf_get_stats");
+      vcom = st.vcom;
+      if (energy_row(a, true)) return 1;
+    }
+    if (a % o.M == 0) {  // src/md.cpp:647-669
+      sf_get_state(h, pos.data(), nullptr);
+      ofile << "ITEM: TIMESTEP" << std::endl << a << std::endl << "ITEM: NUMBER OF ATOMS" << std::endl << N << std::endl
+            << "ITEM: BOX BOUNDS" << std::endl << -prm.box / 2 << std::setw(15) << prm.box / 2 << std::endl << -prm.box / 2
+            << std::setw(15) << prm.box / 2 << std::endl << -prm.box / 2 << std::setw(15) << prm.box / 2 << std::endl
+            << "ITEM: ATOMS index type x y z b charge" << std::endl;
+      for (long long b = 0; b < N; b++)
+        ofile << b + 1 << std::setw(15) << tb.type[b % np] << std::setw(15) << pos[3 * b] << std::setw(15) << pos[3 * b + 1]
+              << std::setw(15) << pos[3 * b + 2] << std::setw(15) << per_bead[5 * b + 1] << std::setw(15) << tb.q[b % np]
+              << std::endl;
+    }
+    const double percentage = std::round(a / (totaltime / o.dt) * 100 * 10) / 10;
+    if (percentage != percentagePre) {
+      progress(percentage / 100);
+      percentagePre = percentage;
+    }
+    a++;
+  }
+  std::cout << std::endl;
+  sf_destroy(h);
+  sf_template_free(tmpl);
+  return 0;
+}

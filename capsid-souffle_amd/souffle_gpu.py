@@ -38,7 +38,7 @@ class SfStats(ctypes.Structure):
                 ("n_replicas", ctypes.c_int32), ("k_integrate_ms", ctypes.c_double),
                 ("k_rebuild_ms", ctypes.c_double),
                 ("k_force_ms", ctypes.c_double), ("k_reduce_ms", ctypes.c_double),
-                ("k_timed_steps", ctypes.c_double)]
+                ("k_timed_steps", ctypes.c_double), ("vcom", ctypes.c_double)]
 
 
 # every entry point of include/souffle_gpu.h: name -> (restype, argtypes)

@@ -86,6 +86,7 @@ typedef struct {
   double k_force_ms;
   double k_reduce_ms;
   double k_timed_steps;   /* steps the means were taken over (0 if timing was off) */
+  double vcom;            /* replica 0: |centre-of-mass velocity| removed at the last step */
 } sf_stats;
 
 /* ---- template and parameters ---------------------------------------------------------- */

@@ -1,0 +1,52 @@
+"""The souffle-gpu command-line driver: the reference's flags, outfiles/ writers and restart."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_lib import REPO, TEMPLATE
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(REPO, "capsid-souffle_amd", "bin", "souffle-gpu")
+FLAGS = ["-f", "trimer_MVM", "-S", "20", "-C", "800", "-c", "100", "-E", "2.1", "-t", "0.005", "-K", "298"]
+
+
+def _run(cwd, extra):
+    r = subprocess.run([DRIVER] + FLAGS + extra, cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def _rows(path):
+    rows = []
+    for line in open(path):
+        tok = line.split()
+        if tok and tok[0] != "time":
+            rows.append([float(x) for x in tok])
+    return rows
+
+
+def test_driver_outputs_and_resume(tmp_path):
+    os.makedirs(tmp_path / "infiles")
+    shutil.copy(TEMPLATE, tmp_path / "infiles" / "trimer_MVM")
+    _run(tmp_path, ["-T", "100", "-W", "50", "-w", "50", "-M", "50", "-R", "false"])
+    out = tmp_path / "outfiles"
+    rows = _rows(out / "energy.out")
+    # initial row (11 columns) then loop rows at a = 0, 50, 100 (12 columns)
+    assert [r[0] for r in rows] == [0, 0, 50, 100]
+    assert len(rows[0]) == 11 and all(len(r) == 12 for r in rows[1:])
+    assert abs(rows[0][1] - 1.5) < 1e-4        # KE/N = 3T/2 at T = 1
+    assert abs(rows[0][8] - 1.0) < 1e-4        # temperature column
+    for r in rows:
+        assert np.isfinite(r).all()
+    for a in (0, 50, 100):
+        assert (out / f"restart_{a}.out").exists()
+    assert open(out / "ovito.lammpstrj").read().count("ITEM: TIMESTEP") == 3
+    # resume from the newest restart file (restart_100.out) and run to a = 150
+    _run(tmp_path, ["-T", "150", "-W", "50", "-w", "50", "-M", "1000", "-R", "true"])
+    rows2 = _rows(out / "energy.out")
+    assert [r[0] for r in rows2[len(rows):]] == [0, 100, 150]
+    assert (out / "restart_150.out").exists()
