@@ -63,8 +63,15 @@ def make_system(args, brownian=False, fp32=False, skin=None):
     return s, pos, vel
 
 
+def extended_energy(s):
+    """Sum of the energy file's terms (kinetic, stretch, bend, LJ, Yukawa, thermostat potential
+    and kinetic) per bead: the quantity a Nose-Hoover chain conserves."""
+    return float(s.energies()[0].sum()) / s.n_beads
+
+
 def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
     s, pos, vel = make_system(args, brownian, fp32, skin)
+    e_start = extended_energy(s)
     s.step(args.warmup)
     st0 = s.stats()
     t0 = time.perf_counter()
@@ -77,6 +84,7 @@ def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
                rebuilds=st1["rebuilds"] - st0["rebuilds"],
                n_list=(st1["lj_entries"] + st1["q_entries"]) / nb)
     out["k_rebuild"] = args.steps / max(out["rebuilds"], 1)
+    out["e_start"], out["e_end"] = e_start, extended_energy(s)
     if timed_pass:
         s.set_timing(True)
         s.step(args.steps)
@@ -119,8 +127,14 @@ def probe_copy(mb):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--workload", choices=["md", "replicas"], default="md",
+                    help="md: one 2000-subunit system (default); replicas: the 256-replica sweep")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (md 20000, replicas 3000)")
     ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--tune-steps", type=int, default=100)
+    ap.add_argument("--cpu-replicas", type=int, default=8)
+    ap.add_argument("--cpu-replica-steps", type=int, default=400)
     ap.add_argument("--subunits", type=int, default=2000)
     ap.add_argument("--skin", type=float, default=0.3)
     ap.add_argument("--cpu-steps", type=int, default=150)
@@ -129,6 +143,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the BD and FP32-pair runs")
     ap.add_argument("--probe-copy-mb", type=int, default=0, help="run a known-size copy kernel first")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 3000 if args.workload == "replicas" else 20000
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one large system would shard "
               "into subunit-owning spatial domains with one halo exchange per step).", file=sys.stderr)
@@ -140,12 +156,115 @@ def main():
     if args.probe_copy_mb > 0:
         probe_copy(args.probe_copy_mb)
 
+    if args.workload == "replicas":
+        out = bench_replicas(args)
+    else:
+        out = bench_md(args)
+    print(json.dumps(out), flush=True)
+
+
+def roofline_entry(nb, n_list, k_rebuild, t_force, value, traffic):
+    b_force = force_kernel_bytes_per_bead(n_list)
+    achieved = b_force * nb / (t_force * 1e-3) / 1e9 if t_force > 0 else 0.0
+    b_step = step_bytes_per_bead(n_list, k_rebuild)
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-scaled)",
+            "algorithmic_bytes_per_launch": b_force * nb,
+            "kernel": "k_force", "kernel_ms": t_force, "bytes_per_bead": b_force,
+            "n_list": n_list, "k_rebuild": k_rebuild,
+            "step_bytes_per_bead": b_step, "step_achieved_GBs": b_step * value / 1e9}
+
+
+def bench_replicas(args):
+    """A seeded sweep of uneven replicas in one handle: subunit counts uniform in [20, 400],
+    salt 50-800 mM, LJ depth 1.5-2.5, temperature 280-320 K, C = 800 uM, FP32 pair / FP64
+    integrate, NH MD; skin and gather tile picked by the library's autotuner. value is the
+    aggregate bead-steps/s. A few replicas are rerun alone and compared bit for bit."""
+    import numpy as np
+    import souffle_gpu as sg
+    import replica_sweep
+    from oracle_lib import TEMPLATE, Oracle
+    t = sg.Template(TEMPLATE)
+    sizes, params, pos, vel, draws = replica_sweep.sweep(t, args.replicas, 4)
+    batch = sg.System(t, sizes, params)
+    batch.set_state(np.concatenate(pos), np.concatenate(vel))
+    tune, trial_ms = batch.autotune(trial_steps=args.tune_steps)
+    batch.step(args.warmup)
+    st0 = batch.stats()
+    t0 = time.perf_counter()
+    batch.step(args.steps)
+    wall = time.perf_counter() - t0
+    st1 = batch.stats()
+    nb = batch.n_beads
+    value = nb * args.steps / wall
+    n_list = (st1["lj_entries"] + st1["q_entries"]) / nb
+    rebuilds_per_rep = (st1["rebuilds"] - st0["rebuilds"]) / len(sizes)
+    k_rebuild = args.steps / max(rebuilds_per_rep, 1)
+    bp, bv = batch.get_state()
+    # bit-for-bit check of the smallest, largest, lowest-salt and highest-salt replicas
+    salts = [d["c_mM"] for d in draws]
+    picks = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), int(np.argmin(salts)), int(np.argmax(salts))})
+    offs = np.concatenate([[0], np.cumsum(sizes)]) * t.n_beads
+    checked = []
+    for k in picks:
+        single = sg.System(t, sizes[k], params[k])
+        single.set_state(pos[k], vel[k])
+        single.step(args.warmup + args.steps)
+        sp, sv = single.get_state()
+        same = bool(np.array_equal(bp[offs[k]:offs[k + 1]], sp) and np.array_equal(bv[offs[k]:offs[k + 1]], sv))
+        checked.append({"replica": k, "subunits": sizes[k], "c_mM": round(salts[k], 1), "bitwise_equal": same})
+        single.close()
+    batch.set_timing(True)
+    batch.step(min(args.steps, 1000))
+    batch.set_timing(False)
+    st = batch.stats()
+    out = {
+        "metric": "bead-steps/s", "value": value, "unit": "bead-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pairs, f64 integrate", "data": "synthetic",
+        "config": {"workload": f"{len(sizes)} replicas of trimer_MVM, subunits U[20,400] (total {sum(sizes)}), "
+                               f"c U[50,800] mM, E U[1.5,2.5], K U[280,320], C=800uM dt=0.005, NH q=5, "
+                               f"FP32 pair arithmetic, autotuned skin/tile",
+                   "replicas": len(sizes), "subunits": sum(sizes), "beads": nb, "parallelism": "single GPU"},
+        "roofline": roofline_entry(nb, n_list, k_rebuild, st["k_force_ms"], value, None),
+        "kernels_ms": {"k_integrate": st["k_integrate_ms"], "k_rebuild": st["k_rebuild_ms"],
+                       "k_force": st["k_force_ms"], "k_finalize": st["k_reduce_ms"]},
+        "autotune": {"skin": tune["skin"], "tile": tune["tile"], "trial_ms_per_step": trial_ms.round(4).tolist(),
+                     "skins": [0.2, 0.3, 0.5, 0.8, 1.2], "tiles": [128, 256, 512]},
+        "single_run_check": checked,
+    }
+    if not args.no_cpu:
+        # the oracle on a seeded sample of the replicas, one after another on all host cores
+        # (the sequential parameter-sweep equivalent), each at its smallest valid -L, -B 5
+        rng = np.random.default_rng(9)
+        sample = sorted(rng.choice(len(sizes), size=min(args.cpu_replicas, len(sizes)), replace=False).tolist())
+        threads = os.cpu_count() or 1
+        if args.cpu_threads > 0:
+            threads = min(threads, args.cpu_threads)
+        work = secs = 0.0
+        for k in sample:
+            p, d = params[k], draws[k]
+            L = replica_sweep.reference_list_radius(TEMPLATE, p.ecut, p.ecut_el)
+            o = Oracle(sizes[k], C_uM=800.0, c_mM=d["c_mM"], elj_att=d["elj_att"], K=d["K"], dt=0.005,
+                       Q=10.0, q_chain=5, L=L, B=5)
+            o.set_state(pos[k], vel[k])
+            o.time_steps(1, threads)
+            secs += o.time_steps(args.cpu_replica_steps, threads)
+            work += o.N * args.cpu_replica_steps
+            o.close()
+        out["cpu_baseline"] = {"value": work / secs, "unit": "bead-steps/s", "cores": threads, "kind": "port",
+                               "sample": f"{len(sample)} of the replicas, {args.cpu_replica_steps} NH steps each, "
+                                         f"sequentially (oracle, FP64, reference list algorithm at each replica's "
+                                         f"smallest valid -L, -B 5), {secs:.1f} s"}
+    return out
+
+
+def bench_md(args):
     md = run_gpu(args)
     st = md["stats"]
     t_force = st["k_force_ms"]
-    b_force = force_kernel_bytes_per_bead(md["n_list"])
-    achieved = b_force * md["nb"] / (t_force * 1e-3) / 1e9 if t_force > 0 else 0.0
-    b_step = step_bytes_per_bead(md["n_list"], md["k_rebuild"])
     out = {
         "metric": "bead-steps/s",
         "value": md["value"],
@@ -162,14 +281,8 @@ def main():
         "config": {"workload": f"trimer_MVM x{args.subunits} plain FP64 Nose-Hoover MD (q=5, Q=10), "
                                f"C=800uM c=100mM E=2.1 dt=0.005 K=298 ks=50 kb=20, skin {args.skin}",
                    "subunits": args.subunits, "beads": md["nb"], "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
-                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-scaled)",
-                     "algorithmic_bytes_per_launch": b_force * md["nb"],
-                     "kernel": "k_force", "kernel_ms": t_force, "bytes_per_bead": b_force,
-                     "n_list": md["n_list"], "k_rebuild": md["k_rebuild"],
-                     "step_bytes_per_bead": b_step,
-                     "step_achieved_GBs": b_step * md["value"] / 1e9},
+        "roofline": roofline_entry(md["nb"], md["n_list"], md["k_rebuild"], t_force, md["value"],
+                                   measured_traffic(args)),
         "kernels_ms": {"k_integrate": st["k_integrate_ms"], "k_rebuild": st["k_rebuild_ms"],
                        "k_force": t_force, "k_finalize": st["k_reduce_ms"]},
     }
@@ -184,9 +297,17 @@ def main():
         f64 = md["system"].forces()[0]
         ff = f32["system"].forces()[0]
         rms = float(np.sqrt(np.sum((ff - f64) ** 2) / np.sum(f64 ** 2)))
+        n_run = args.warmup + args.steps
         out["extra"] = {"bd_bead_steps_per_s": bd["value"], "bd_ms_per_step": bd["ms_per_step"],
                         "fp32_pair_bead_steps_per_s": f32["value"], "fp32_pair_ms_per_step": f32["ms_per_step"],
-                        "fp32_pair_force_rel_rms_err": rms}
+                        "fp32_pair_force_rel_rms_err": rms,
+                        # extended NH energy per bead (kT) at the start and after warmup+steps, from
+                        # the same seeded state
+                        "energy_drift_steps": n_run,
+                        "fp64_energy_per_bead": [md["e_start"], md["e_end"]],
+                        "fp32_pair_energy_per_bead": [f32["e_start"], f32["e_end"]],
+                        "fp64_energy_drift_per_bead": md["e_end"] - md["e_start"],
+                        "fp32_pair_energy_drift_per_bead": f32["e_end"] - f32["e_start"]}
     for k in ("system",):
         md.pop(k, None)
     if not args.no_cpu:
@@ -196,7 +317,7 @@ def main():
                                "sample": f"{args.cpu_steps} NH steps of the same {args.subunits}-subunit state "
                                          f"(oracle, FP64, the reference list algorithm at -L 0.3 -B 5), "
                                          f"{cpu['secs']:.1f} s"}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -96,6 +97,31 @@ struct sf_handle {
       // inside half a box; bead ids must fit the 2^23 float index of the grid records
       P.grid_ok = (rb + rlj + 2e-3 < 0.5 * p.box && (long long)Nb < (1ll << 23)) ? 1 : 0;
     }
+  }
+
+  // largest skin replica r can use: minimum image needs max(cutoffs) + skin < box/2
+  double max_skin(int r) const {
+    const sf_params& p = prm[r];
+    return 0.5 * p.box - std::max(p.ecut, p.ecut_el) - 1e-9;
+  }
+
+  // Sets replica r's Verlet skin (host tables only; upload_rp and a forced rebuild follow).
+  void put_skin(int r, double skin) {
+    prm[r].skin = skin;
+    RepParams& P = rp[r];
+    P.skin = skin;
+    P.half_skin2 = 0.25 * skin * skin;
+    P.rc_q_list2 = (prm[r].ecut_el + skin) * (prm[r].ecut_el + skin);
+  }
+
+  // Marks every replica for a list build at the next step.
+  int request_rebuild() {
+    std::vector<RepState> h;
+    if (int rc = read_states(h)) return rc;
+    for (auto& s : h) s.rebuild = 1;
+    HIPCHK(hipMemcpyAsync(d.rs, h.data(), R * sizeof(RepState), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return SF_OK;
   }
 
   int upload_rp() {
@@ -228,6 +254,7 @@ struct sf_handle {
   bool g_tm = false;
   const void* g_key[2] = {nullptr, nullptr};
   int g_cap[2] = {0, 0};
+  int g_tile = 0;
   std::vector<hipEvent_t> events;
 
   void drop_graph() {
@@ -252,7 +279,7 @@ struct sf_handle {
     // the captured kernel arguments hold the list buffers and their capacities; a regrown
     // buffer may come back at the same address, so the capacities are part of the key
     if (!gexec || g_n != n || g_tm != timing || g_key[0] != d.lj_list || g_key[1] != d.q_list ||
-        g_cap[0] != d.lj_cap || g_cap[1] != d.q_cap) {
+        g_cap[0] != d.lj_cap || g_cap[1] != d.q_cap || g_tile != d.tile) {
       drop_graph();
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -264,7 +291,7 @@ struct sf_handle {
       (void)hipGraphDestroy(g);
       if (ei != hipSuccess) { gexec = nullptr; set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return SF_E_HIP; }
       g_n = n; g_tm = timing; g_key[0] = d.lj_list; g_key[1] = d.q_list;
-      g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap;
+      g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap; g_tile = d.tile;
     }
     HIPCHK(hipGraphLaunch(gexec, st));
     return SF_OK;
@@ -316,6 +343,7 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
   if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed (no GPU?)");
   Dev& d = h->d;
   d.Nb = h->Nb; d.Ns = h->Ns; d.Nq = h->Nq; d.R = R;
+  d.tile = 256;
   // template tables
   TopoDev& td = d.topo;
   td.np = np; td.nh = t->nh(); td.nq = nq; td.centre = t->centre;
@@ -637,6 +665,111 @@ int sf_bd_uniforms(sf_handle* h, int replica, int64_t step, double* out) {
   }
   h->release(dout);
   return rc;
+}
+
+int sf_set_tile(sf_handle* h, int tile) {
+  if (!h) { set_error("null handle"); return SF_E_ARG; }
+  if (!tile_supported(tile)) { set_error("tile must be 128, 256 or 512"); return SF_E_ARG; }
+  h->d.tile = tile;
+  return SF_OK;
+}
+
+int sf_set_skin(sf_handle* h, int replica, double skin) {
+  if (!h || replica < -1 || replica >= h->R) { set_error("bad replica"); return SF_E_ARG; }
+  if (!(skin > 0)) { set_error("skin must be positive"); return SF_E_ARG; }
+  const int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? h->R : replica + 1;
+  for (int r = r0; r < r1; r++)
+    if (skin > h->max_skin(r)) {
+      set_error("box too small for this skin (minimum image needs cutoff + skin < box/2)");
+      return SF_E_ARG;
+    }
+  for (int r = r0; r < r1; r++) h->put_skin(r, skin);
+  h->fill_bounds();
+  if (int rc = h->upload_rp()) return rc;
+  return h->request_rebuild();
+}
+
+int sf_get_skin(sf_handle* h, int replica, double* skin) {
+  if (!h || replica < 0 || replica >= h->R || !skin) { set_error("bad arguments"); return SF_E_ARG; }
+  *skin = h->prm[replica].skin;
+  return SF_OK;
+}
+
+int sf_autotune(sf_handle* h, const sf_tune_opts* opt, sf_tune_result* res, double* trial_ms) {
+  if (!h || !res) { set_error("null argument"); return SF_E_ARG; }
+  static const double kSkins[] = {0.2, 0.3, 0.5, 0.8, 1.2};
+  static const int kTiles[] = {128, 256, 512};
+  const double* skins = kSkins;
+  const int* tiles = kTiles;
+  int ns = 5, nt = 3, trial = 200;
+  if (opt) {
+    if (opt->n_skins > 0) { if (!opt->skins) { set_error("skins is null"); return SF_E_ARG; } skins = opt->skins; ns = opt->n_skins; }
+    if (opt->n_tiles > 0) { if (!opt->tiles) { set_error("tiles is null"); return SF_E_ARG; } tiles = opt->tiles; nt = opt->n_tiles; }
+    if (opt->trial_steps > 0) trial = opt->trial_steps;
+  }
+  for (int i = 0; i < ns; i++)
+    if (!(skins[i] > 0)) { set_error("candidate skins must be positive"); return SF_E_ARG; }
+  for (int j = 0; j < nt; j++)
+    if (!tile_supported(tiles[j])) { set_error("candidate tiles must be 128, 256 or 512"); return SF_E_ARG; }
+  // save the dynamic state; every trial starts from it and the caller gets it back
+  const size_t nb = h->Nb;
+  double4* a_pos = nullptr; double* a_v = nullptr; double* a_f = nullptr; double* a_g = nullptr;
+  RepState* a_rs = nullptr;
+  if (h->alloc(&a_pos, nb) || h->alloc(&a_v, 3 * nb) || h->alloc(&a_f, 3 * nb) || h->alloc(&a_g, 3 * nb) ||
+      h->alloc(&a_rs, h->R)) {
+    h->release(a_pos); h->release(a_v); h->release(a_f); h->release(a_g); h->release(a_rs);
+    return SF_E_HIP;
+  }
+  auto copy_state = [&](bool save) -> int {
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    Dev& d = h->d;
+    HIPCHK(hipMemcpyAsync(save ? (void*)a_pos : (void*)d.pos, save ? (void*)d.pos : (void*)a_pos, nb * sizeof(double4), k, h->st));
+    HIPCHK(hipMemcpyAsync(save ? a_v : d.vx, save ? d.vx : a_v, 3 * nb * sizeof(double), k, h->st));
+    HIPCHK(hipMemcpyAsync(save ? a_f : d.fx, save ? d.fx : a_f, 3 * nb * sizeof(double), k, h->st));
+    HIPCHK(hipMemcpyAsync(save ? a_g : d.gx, save ? d.gx : a_g, 3 * nb * sizeof(double), k, h->st));
+    HIPCHK(hipMemcpyAsync(save ? (void*)a_rs : (void*)d.rs, save ? (void*)d.rs : (void*)a_rs, h->R * sizeof(RepState), k, h->st));
+    if (!save)
+      if (int rc = launch_centres(d, h->st)) return rc;
+    HIPCHK(hipStreamSynchronize(h->st));
+    return SF_OK;
+  };
+  // skin per replica: the candidate, or the replica's largest valid skin when its box is smaller
+  auto apply = [&](double skin, int tile) -> int {
+    for (int r = 0; r < h->R; r++) h->put_skin(r, std::min(skin, h->max_skin(r)));
+    h->fill_bounds();
+    h->d.tile = tile;
+    if (int rc = h->upload_rp()) return rc;
+    return h->request_rebuild();
+  };
+  const bool timing = h->timing;
+  const double t_save[5] = {h->t_int, h->t_rb, h->t_force, h->t_red, h->t_steps};
+  h->timing = false;
+  int rc = copy_state(true);
+  double best = 1e300;
+  int bi = 0, bj = 0, n = 0;
+  for (int i = 0; i < ns && !rc; i++)
+    for (int j = 0; j < nt && !rc; j++) {
+      if ((rc = copy_state(false)) || (rc = apply(skins[i], tiles[j]))) break;
+      // one chunk first: graph capture and list-capacity growth stay out of the timed steps
+      if ((rc = sf_step(h, kChunk))) break;
+      auto t0 = std::chrono::steady_clock::now();
+      if ((rc = sf_step(h, trial))) break;
+      const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / trial;
+      if (trial_ms) trial_ms[i * nt + j] = ms;
+      n++;
+      if (ms < best) { best = ms; bi = i; bj = j; }
+    }
+  if (!rc) rc = copy_state(false);
+  if (!rc) rc = apply(skins[bi], tiles[bj]);
+  h->timing = timing;
+  h->t_int = t_save[0]; h->t_rb = t_save[1]; h->t_force = t_save[2]; h->t_red = t_save[3]; h->t_steps = t_save[4];
+  h->release(a_pos); h->release(a_v); h->release(a_f); h->release(a_g); h->release(a_rs);
+  if (rc) return rc;
+  res->skin = skins[bi];
+  res->tile = tiles[bj];
+  res->ms_per_step = best;
+  res->n_trials = n;
+  return SF_OK;
 }
 
 }  // extern "C"

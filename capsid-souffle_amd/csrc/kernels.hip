@@ -574,7 +574,6 @@ __device__ __forceinline__ double hinge_forces(const double4* own, int4 hg, doub
   return c;
 }
 
-constexpr int kChunk = 256;  // list entries a workgroup evaluates per gather round
 
 // Per-template-bead tables a workgroup keeps in LDS for the neighbour lookups.
 struct BeadLds {
@@ -618,12 +617,13 @@ __device__ __forceinline__ PT yk_term(PT r2, PT qq, PT yk, PT kappa, PT ecut_el2
 }
 
 // LDS bytes of k_force: own positions, hinge forces, Yukawa tables of the charged beads, and
-// the pair terms of one gather round.
-static inline size_t force_lds_bytes(int np, int nh, int nq) {
-  return (size_t)np * 32 + (size_t)nh * 96 + (size_t)nq * nq * 16 + (size_t)kChunk * 3 * 8;
+// the pair terms of one gather round of `tile` entries.
+static inline size_t force_lds_bytes(int np, int nh, int nq, int tile) {
+  return (size_t)np * 32 + (size_t)nh * 96 + (size_t)nq * nq * 16 + (size_t)tile * 3 * 8;
 }
 
-template <int MODE, typename PT>
+// kChunk: list entries the workgroup evaluates per gather round (the tile).
+template <int MODE, typename PT, int kChunk>
 __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
   extern __shared__ double smem[];
   __shared__ double red[2][kPartStride];
@@ -1075,18 +1075,28 @@ int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st) {
   return check_launch();
 }
 
-int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st) {
+bool tile_supported(int tile) { return tile == 128 || tile == 256 || tile == 512; }
+
+template <int CH>
+static void launch_force_tile(const Dev& d, int mode, int init, int fp32, size_t sm, hipStream_t st) {
   dim3 g(d.Ns), b(kBlock);
-  const size_t sm = force_lds_bytes(d.topo.np, d.topo.nh, d.topo.nq);
   if (fp32) {
-    if (mode == MODE_FORCES) hipLaunchKernelGGL((k_force<MODE_FORCES, float>), g, b, sm, st, d, init);
-    else if (mode == MODE_MD) hipLaunchKernelGGL((k_force<MODE_MD, float>), g, b, sm, st, d, init);
-    else hipLaunchKernelGGL((k_force<MODE_BD, float>), g, b, sm, st, d, init);
+    if (mode == MODE_FORCES) hipLaunchKernelGGL((k_force<MODE_FORCES, float, CH>), g, b, sm, st, d, init);
+    else if (mode == MODE_MD) hipLaunchKernelGGL((k_force<MODE_MD, float, CH>), g, b, sm, st, d, init);
+    else hipLaunchKernelGGL((k_force<MODE_BD, float, CH>), g, b, sm, st, d, init);
   } else {
-    if (mode == MODE_FORCES) hipLaunchKernelGGL((k_force<MODE_FORCES, double>), g, b, sm, st, d, init);
-    else if (mode == MODE_MD) hipLaunchKernelGGL((k_force<MODE_MD, double>), g, b, sm, st, d, init);
-    else hipLaunchKernelGGL((k_force<MODE_BD, double>), g, b, sm, st, d, init);
+    if (mode == MODE_FORCES) hipLaunchKernelGGL((k_force<MODE_FORCES, double, CH>), g, b, sm, st, d, init);
+    else if (mode == MODE_MD) hipLaunchKernelGGL((k_force<MODE_MD, double, CH>), g, b, sm, st, d, init);
+    else hipLaunchKernelGGL((k_force<MODE_BD, double, CH>), g, b, sm, st, d, init);
   }
+}
+
+int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st) {
+  const size_t sm = force_lds_bytes(d.topo.np, d.topo.nh, d.topo.nq, d.tile);
+  if (d.tile == 128) launch_force_tile<128>(d, mode, init, fp32, sm, st);
+  else if (d.tile == 512) launch_force_tile<512>(d, mode, init, fp32, sm, st);
+  else if (d.tile == 256) launch_force_tile<256>(d, mode, init, fp32, sm, st);
+  else { set_error("unsupported k_force tile"); return SF_E_ARG; }
   return check_launch();
 }
 

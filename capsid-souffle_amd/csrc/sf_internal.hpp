@@ -144,6 +144,7 @@ struct Dev {
   double* out_f; double* out_s; double* out_b; double* out_e; double* out_lj;  // [Nb][3], nullable
   int lj_cap, q_cap;
   int Nb, Ns, Nq, R;
+  int tile;            // list entries k_force stages in LDS per gather round (128, 256, 512)
 };
 
 enum { MODE_FORCES = 0, MODE_MD = 1, MODE_BD = 2 };
@@ -152,7 +153,8 @@ enum { MODE_FORCES = 0, MODE_MD = 1, MODE_BD = 2 };
 int launch_centres(const Dev& d, hipStream_t st);
 int launch_integrate(const Dev& d, int mode, hipStream_t st);
 int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st);
-int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st);
+int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st);  // uses d.tile
+bool tile_supported(int tile);
 int launch_finalize(const Dev& d, int init, hipStream_t st);
 int launch_apply_com(const Dev& d, hipStream_t st);
 int launch_energy(const Dev& d, hipStream_t st);

@@ -41,6 +41,16 @@ class SfStats(ctypes.Structure):
                 ("k_timed_steps", ctypes.c_double), ("vcom", ctypes.c_double)]
 
 
+class SfTuneOpts(ctypes.Structure):
+    _fields_ = [("trial_steps", ctypes.c_int), ("n_skins", ctypes.c_int), ("skins", ctypes.POINTER(ctypes.c_double)),
+                ("n_tiles", ctypes.c_int), ("tiles", ctypes.POINTER(ctypes.c_int))]
+
+
+class SfTuneResult(ctypes.Structure):
+    _fields_ = [("skin", ctypes.c_double), ("tile", ctypes.c_int), ("ms_per_step", ctypes.c_double),
+                ("n_trials", ctypes.c_int)]
+
+
 # every entry point of include/souffle_gpu.h: name -> (restype, argtypes)
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
@@ -63,6 +73,10 @@ SIGNATURES = {
     "sf_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "sf_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(SfStats)]),
     "sf_bd_uniforms": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _D]),
+    "sf_set_skin": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double]),
+    "sf_get_skin": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "sf_set_tile": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sf_autotune": (ctypes.c_int, [_P, ctypes.POINTER(SfTuneOpts), ctypes.POINTER(SfTuneResult), _D]),
     "sf_last_error": (ctypes.c_char_p, []),
 }
 
@@ -209,6 +223,29 @@ class System:
         out = np.zeros((nb, 3))
         _err(_lib.sf_bd_uniforms(self.ptr, replica, int(step), _dp(out)), "sf_bd_uniforms")
         return out
+
+    def set_skin(self, skin, replica=-1):
+        _err(_lib.sf_set_skin(self.ptr, int(replica), float(skin)), "sf_set_skin")
+
+    def skin(self, replica=0):
+        v = ctypes.c_double()
+        _err(_lib.sf_get_skin(self.ptr, int(replica), ctypes.byref(v)), "sf_get_skin")
+        return v.value
+
+    def set_tile(self, tile):
+        _err(_lib.sf_set_tile(self.ptr, int(tile)), "sf_set_tile")
+
+    def autotune(self, skins=None, tiles=None, trial_steps=0):
+        """Runs the library's autotuner; returns (result dict, trial ms/step [n_skins][n_tiles])."""
+        o = SfTuneOpts(trial_steps=int(trial_steps))
+        sk = np.ascontiguousarray(skins if skins is not None else [0.2, 0.3, 0.5, 0.8, 1.2], dtype=np.float64)
+        tl = np.ascontiguousarray(tiles if tiles is not None else [128, 256, 512], dtype=np.int32)
+        o.n_skins, o.skins = len(sk), sk.ctypes.data_as(_D)
+        o.n_tiles, o.tiles = len(tl), tl.ctypes.data_as(_I)
+        res = SfTuneResult()
+        ms = np.zeros((len(sk), len(tl)))
+        _err(_lib.sf_autotune(self.ptr, ctypes.byref(o), ctypes.byref(res), _dp(ms)), "sf_autotune")
+        return {k: getattr(res, k) for k, _ in SfTuneResult._fields_}, ms
 
     def close(self):
         if getattr(self, "ptr", None):

@@ -161,6 +161,41 @@ int sf_get_stats(sf_handle* h, sf_stats* out);
  * (src/md.cpp:382-385, 464-466), computed on the GPU by skip-ahead. */
 int sf_bd_uniforms(sf_handle* h, int replica, int64_t step, double* out);
 
+/* ---- tuning (no reference counterpart: the reference's list is rebuilt every -B steps with
+ * radius -L, src/md.cpp:438-443; here the list is a Verlet list with a displacement trigger) -- */
+
+/* Verlet skin of one replica (replica >= 0) or of all (-1). The next step rebuilds the lists.
+ * Forces do not depend on the skin (lists hold every pair inside the cutoffs either way), so
+ * the trajectory is unchanged bit for bit; only the rebuild frequency and list lengths are. */
+int sf_set_skin(sf_handle* h, int replica, double skin);
+int sf_get_skin(sf_handle* h, int replica, double* skin);
+
+/* Pair-gather tile of the force kernel: list entries a subunit workgroup evaluates per round
+ * (128, 256 or 512). Results are the same bit for bit for every tile. */
+int sf_set_tile(sf_handle* h, int tile);
+
+typedef struct {
+  int trial_steps;       /* timed steps per candidate; <= 0: 200 */
+  int n_skins;           /* candidate skins; 0: {0.2, 0.3, 0.5, 0.8, 1.2} */
+  const double* skins;
+  int n_tiles;           /* candidate tiles; 0: {128, 256, 512} */
+  const int* tiles;
+} sf_tune_opts;
+
+typedef struct {
+  double skin;           /* chosen skin (a replica whose box cannot take it gets its largest valid one) */
+  int tile;              /* chosen tile */
+  double ms_per_step;    /* wall time per step of the chosen pair in its trial */
+  int n_trials;
+} sf_tune_result;
+
+/* Autotuner: from the current state, runs every (skin, tile) candidate for one untimed
+ * 100-step chunk and then trial_steps timed steps (wall clock, HIP graphs as in sf_step),
+ * restores the state after each trial, and leaves the fastest pair set. The state, the
+ * step counter and the trajectory that follows are exactly as without the call.
+ * opts may be NULL (defaults); trial_ms (nullable) receives [n_skins][n_tiles] ms/step. */
+int sf_autotune(sf_handle* h, const sf_tune_opts* opts, sf_tune_result* out, double* trial_ms);
+
 const char* sf_last_error(void);
 
 #ifdef __cplusplus

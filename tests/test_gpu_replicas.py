@@ -1,0 +1,92 @@
+"""Batched parameter-sweep replicas, the Verlet skin and gather tile, and the autotuner.
+
+Forces are summed per bead in list order over lists that hold every pair inside the cutoffs,
+so the skin, the tile, the tuner's trials and the batching must leave every trajectory
+unchanged bit for bit.
+"""
+import numpy as np
+import pytest
+
+import souffle_gpu as sg
+import replica_sweep
+from oracle_lib import TEMPLATE
+
+pytestmark = pytest.mark.gpu
+
+
+def sweep(n, seed, ns_range=(20, 60), fp32=True):
+    t = sg.Template(TEMPLATE)
+    sizes, params, pos, vel, _ = replica_sweep.sweep(t, n, seed, ns_range=ns_range, fp32=fp32)
+    return t, sizes, params, pos, vel
+
+
+def test_skin_and_tile_leave_trajectory_unchanged():
+    t, sizes, params, pos, vel = sweep(1, 3, (30, 30), fp32=False)
+    a = sg.System(t, sizes, params)
+    a.set_state(pos[0], vel[0])
+    a.step(150)
+    ap, av = a.get_state()
+    b = sg.System(t, sizes, params)
+    b.set_state(pos[0], vel[0])
+    b.step(50)
+    b.set_skin(1.0)
+    b.set_tile(128)
+    b.step(50)
+    assert b.skin() == 1.0
+    b.set_skin(0.15)
+    b.set_tile(512)
+    b.step(50)
+    bp, bv = b.get_state()
+    assert np.array_equal(ap, bp) and np.array_equal(av, bv)
+    with pytest.raises(sg.SfError):
+        b.set_tile(64)
+    with pytest.raises(sg.SfError):
+        b.set_skin(params[0].box)  # cutoff + skin beyond half a box
+
+
+def test_autotune_keeps_state_and_trajectory():
+    t, sizes, params, pos, vel = sweep(3, 5)
+    x0, v0 = np.concatenate(pos), np.concatenate(vel)
+    a = sg.System(t, sizes, params)
+    a.set_state(x0, v0)
+    a.step(120)
+    ap, av = a.get_state()
+    b = sg.System(t, sizes, params)
+    b.set_state(x0, v0)
+    res, ms = b.autotune(skins=[0.3, 0.8], tiles=[128, 512], trial_steps=40)
+    assert res["n_trials"] == 4 and ms.shape == (2, 2) and (ms > 0).all()
+    assert res["skin"] in (0.3, 0.8) and res["tile"] in (128, 512)
+    assert abs(res["ms_per_step"] - ms.min()) < 1e-12
+    assert b.stats()["steps"] == 0
+    p0, v0b = b.get_state()
+    assert np.array_equal(p0, x0) and np.array_equal(v0b, v0)
+    b.step(120)
+    bp, bv = b.get_state()
+    assert np.array_equal(ap, bp) and np.array_equal(av, bv)
+    assert b.stats()["steps"] == 120
+
+
+def test_sweep_batch_matches_single_runs():
+    """Sixteen uneven replicas of a salt / epsilon / temperature sweep in one handle (FP32 pair
+    arithmetic, autotuned) against the same replicas run one by one."""
+    t, sizes, params, pos, vel = sweep(16, 11)
+    batch = sg.System(t, sizes, params)
+    batch.set_state(np.concatenate(pos), np.concatenate(vel))
+    batch.autotune(skins=[0.3, 0.6], tiles=[256], trial_steps=30)
+    batch.step(200)
+    bp, bv = batch.get_state()
+    assert np.isfinite(bp).all()
+    off = 0
+    for k, ns in enumerate(sizes):
+        single = sg.System(t, ns, params[k])
+        single.set_state(pos[k], vel[k])
+        single.step(200)
+        sp, sv = single.get_state()
+        n = ns * t.n_beads
+        assert np.array_equal(bp[off:off + n], sp), k
+        assert np.array_equal(bv[off:off + n], sv), k
+        # each replica keeps its own thermostat
+        xb, _ = batch.thermostat(k)
+        xs, _ = single.thermostat(0)
+        assert np.array_equal(xb, xs), k
+        off += n
