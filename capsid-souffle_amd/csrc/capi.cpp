@@ -28,6 +28,7 @@ namespace {
 constexpr int kChunk = 100;          // steps between capacity checks
 constexpr int kTimeEvery = 10;       // with timing on, every 10th step's kernels get HIP events
 constexpr double kBoundMargin = 1.0; // initial slack of the centre-bead radius (sigma)
+constexpr double kNoCut2 = 1e30;     // squared "cutoff" of the uncut Yukawa's bead tests
 
 }  // namespace
 
@@ -41,6 +42,7 @@ struct sf_handle {
   std::vector<void*> allocs;
   Dev d{};
   double margin = kBoundMargin;
+  std::vector<double> far_rf;  // per replica, split mode
   int mode = MODE_MD;
   int fp32 = 0;
   bool timing = false;
@@ -90,6 +92,9 @@ struct sf_handle {
       P.sub_rc_lj2 = (rlj + 2 * rb) * (rlj + 2 * rb);
       const double rql = p.ecut_el + p.skin;
       P.sub_rc_q2 = (rql + 2 * rq) * (rql + 2 * rq);
+      // uncut Yukawa: the near lists hold every charged bead of the subunits whose centre
+      // beads are within far_rf + skin (centre beads move at most skin/2 between builds)
+      if (P.split) P.sub_rc_q2 = (far_rf[r] + p.skin) * (far_rf[r] + p.skin);
       P.r_bound = rb;
       P.r_bound2 = rb * rb;
       P.r_bound_q2 = rq * rq;
@@ -102,7 +107,7 @@ struct sf_handle {
   // largest skin replica r can use: minimum image needs max(cutoffs) + skin < box/2
   double max_skin(int r) const {
     const sf_params& p = prm[r];
-    return 0.5 * p.box - std::max(p.ecut, p.ecut_el) - 1e-9;
+    return 0.5 * p.box - std::max(std::max(p.ecut, p.ecut_el), far_rf[r]) - 1e-9;
   }
 
   // Sets replica r's Verlet skin (host tables only; upload_rp and a forced rebuild follow).
@@ -111,7 +116,7 @@ struct sf_handle {
     RepParams& P = rp[r];
     P.skin = skin;
     P.half_skin2 = 0.25 * skin * skin;
-    P.rc_q_list2 = (prm[r].ecut_el + skin) * (prm[r].ecut_el + skin);
+    P.rc_q_list2 = P.split ? kNoCut2 : (prm[r].ecut_el + skin) * (prm[r].ecut_el + skin);
   }
 
   // Marks every replica for a list build at the next step.
@@ -122,6 +127,12 @@ struct sf_handle {
     HIPCHK(hipMemcpyAsync(d.rs, h.data(), R * sizeof(RepState), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     return SF_OK;
+  }
+
+  // far field of the uncut Yukawa (split mode only)
+  int launch_far(int energy) {
+    if (!d.split) return SF_OK;
+    return launch_far_exact(d, energy, st);
   }
 
   int upload_rp() {
@@ -214,6 +225,7 @@ struct sf_handle {
     for (int attempt = 0; attempt < 16; attempt++) {
       if (int rc = launch_centres(d, st)) return rc;
       if (int rc = launch_rebuild(d, 1, st)) return rc;
+      if (int rc = launch_far(0)) return rc;
       if (int rc = launch_force(d, MODE_FORCES, 1, fp32, st)) return rc;
       if (int rc = launch_finalize(d, 1, st)) return rc;
       int rc = check_and_fix();
@@ -238,6 +250,7 @@ struct sf_handle {
       if (int rc = launch_integrate(d, mode, st)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[1], st));
       if (int rc = launch_rebuild(d, 0, st)) return rc;
+      if (int rc = launch_far(0)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[2], st));
       if (int rc = launch_force(d, mode, 0, fp32, st)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[3], st));
@@ -309,13 +322,22 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
   }
   int brown = params[0].brownian;
   int fp32 = params[0].pair_fp32;
+  const int split = params[0].ecut_el == 0 ? 1 : 0;
   long long Ns = 0;
   for (int r = 0; r < R; r++) {
     const sf_params& p = params[r];
     if (n_subunits[r] <= 0) { set_error("replica with no subunits"); return nullptr; }
     if (p.brownian != brown || p.pair_fp32 != fp32) { set_error("all replicas must share the integrator and pair precision"); return nullptr; }
     if (!(p.box > 0) || !(p.dt > 0) || !(p.skin > 0) || !(p.kappa > 0)) { set_error("box, dt, skin and kappa must be positive"); return nullptr; }
-    if (!(p.ecut_el > 0)) { set_error("uncut Yukawa (ecut_c = 0) is not supported by the list engine"); return nullptr; }
+    if (!(p.ecut_el >= 0)) { set_error("ecut_el must be >= 0"); return nullptr; }
+    if ((p.ecut_el == 0) != (split == 1)) { set_error("all replicas must share the Yukawa form (cut or uncut)"); return nullptr; }
+    if (split) {
+      const double rf = p.far_rf > 0 ? p.far_rf : 2.5 * t->diam;
+      if (!(p.far_rf >= 0) || rf + p.skin >= 0.5 * p.box) {
+        set_error("uncut Yukawa: far_rf + skin must be below half the box"); return nullptr;
+      }
+      if (nq > 8) { set_error("uncut Yukawa: the far field supports at most 8 charged beads per template"); return nullptr; }
+    }
     if (!brown && (p.q_chain < 1 || p.q_chain > kMaxChain)) { set_error("Nose-Hoover chain length must be 1..8"); return nullptr; }
     if (brown && !(p.damp > 0)) { set_error("BD damping must be positive"); return nullptr; }
     double reach = std::max(p.ecut, p.ecut_el) + p.skin;
@@ -335,6 +357,8 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
   h->prm.assign(params, params + R);
   h->mode = brown ? MODE_BD : MODE_MD;
   h->fp32 = fp32;
+  h->far_rf.assign(R, 0.0);
+  for (int r = 0; r < R; r++) h->far_rf[r] = split ? (params[r].far_rf > 0 ? params[r].far_rf : 2.5 * t->diam) : 0.0;
   auto fail = [&](const char* what) -> sf_handle* {
     if (what) set_error(what);
     delete h;
@@ -344,6 +368,7 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
   Dev& d = h->d;
   d.Nb = h->Nb; d.Ns = h->Ns; d.Nq = h->Nq; d.R = R;
   d.tile = 256;
+  d.split = split;
   // template tables
   TopoDev& td = d.topo;
   td.np = np; td.nh = t->nh(); td.nq = nq; td.centre = t->centre;
@@ -414,6 +439,9 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
     P.ks = p.ks; P.kb = p.kb; P.dt = p.dt; P.damp = p.damp; P.T = p.T;
     P.skin = p.skin; P.half_skin2 = 0.25 * p.skin * p.skin;
     P.rc_q_list2 = (p.ecut_el + p.skin) * (p.ecut_el + p.skin);
+    P.split = split;
+    P.far_rf2 = h->far_rf[r] * h->far_rf[r];
+    if (split) { P.ecut_el2 = kNoCut2; P.rc_q_list2 = kNoCut2; }
     P.s0 = s0; P.ns = n_subunits[r]; P.b0 = s0 * np; P.nb = n_subunits[r] * np;
     P.q0 = s0 * nq; P.nqr = n_subunits[r] * nq;
     P.inv_nb = 1.0 / (double)P.nb;
@@ -456,14 +484,17 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
       h->alloc(&d.q_cnt, std::max(h->Nq, 1)) || h->alloc(&d.part, (size_t)Ns * kPartStride) ||
       h->alloc(&d.epart, (size_t)Ns * 5) || h->alloc(&d.ebead, nb * 5) ||
       h->alloc(&h->s_pos, nb) || h->alloc(&h->s_v, 3 * nb) || h->alloc(&h->s_f, 3 * nb) ||
-      h->alloc(&h->s_g, 3 * nb) || h->alloc(&h->s_rs, R))
+      h->alloc(&h->s_g, 3 * nb) || h->alloc(&h->s_rs, R) || h->alloc(&d.qpos, std::max(h->Nq, 1)) ||
+      h->alloc(&d.qfar, 3 * (size_t)std::max(h->Nq, 1)) || h->alloc(&d.efar, std::max(h->Nq, 1)))
     return fail(nullptr);
   d.vx = vbuf; d.vy = vbuf + nb; d.vz = vbuf + 2 * nb;
   d.fx = fbuf; d.fy = fbuf + nb; d.fz = fbuf + 2 * nb;
   d.gx = gbuf; d.gy = gbuf + nb; d.gz = gbuf + 2 * nb;
   d.lj_cap = 0; d.q_cap = 0;
   if (h->grow_lists(24, nq > 0 ? 48 : 1)) return fail(nullptr);
-  if (hipMemset(fbuf, 0, 3 * nb * sizeof(double)) != hipSuccess || hipMemset(gbuf, 0, 3 * nb * sizeof(double)) != hipSuccess ||
+  if (hipMemset(d.qfar, 0, 3 * (size_t)std::max(h->Nq, 1) * sizeof(double)) != hipSuccess ||
+      hipMemset(d.efar, 0, (size_t)std::max(h->Nq, 1) * sizeof(double)) != hipSuccess ||
+      hipMemset(fbuf, 0, 3 * nb * sizeof(double)) != hipSuccess || hipMemset(gbuf, 0, 3 * nb * sizeof(double)) != hipSuccess ||
       hipMemset(rsd, 0, R * sizeof(RepState)) != hipSuccess)
     return fail("hipMemset failed");
   return h;
@@ -540,6 +571,7 @@ int sf_forces(sf_handle* h, double* f, double* fs, double* fb, double* fe, doubl
     d.lj_list = h->d.lj_list; d.q_list = h->d.q_list; d.lj_cap = h->d.lj_cap; d.q_cap = h->d.q_cap;
     if (int rc = launch_centres(d, h->st)) return rc;
     if (int rc = launch_rebuild(d, 1, h->st)) return rc;
+    if (int rc = h->launch_far(0)) return rc;
     if (int rc = launch_force(d, MODE_FORCES, 0, h->fp32, h->st)) return rc;
     int rc = h->check_and_fix();
     if (rc < 0) return rc;
@@ -558,6 +590,7 @@ int sf_forces(sf_handle* h, double* f, double* fs, double* fb, double* fe, doubl
 
 int sf_energies(sf_handle* h, double* totals, double* per_bead) {
   if (!h || !totals) { set_error("null argument"); return SF_E_ARG; }
+  if (int rc = h->launch_far(1)) return rc;
   if (int rc = launch_energy(h->d, h->st)) return rc;
   std::vector<double> ep((size_t)h->Ns * 5);
   HIPCHK(hipMemcpyAsync(ep.data(), h->d.epart, ep.size() * sizeof(double), hipMemcpyDeviceToHost, h->st));

@@ -103,7 +103,9 @@ __device__ void chain_xi(int j, double* xi, const double* Qb, const double* dof,
 __global__ void k_centres(Dev d) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.Ns) return;
-  d.centres[s] = d.pos[(size_t)s * d.topo.np + d.topo.centre];
+  const int np = d.topo.np, nq = d.topo.nq;
+  d.centres[s] = d.pos[(size_t)s * np + d.topo.centre];
+  for (int k = 0; k < nq; k++) d.qpos[(size_t)s * nq + k] = d.pos[(size_t)s * np + d.topo.qlocal[k]];
 }
 
 // First half of the step (src/md.cpp:400-431): NH: v = v e + F (dt/2 sqrt(e))/m;
@@ -143,6 +145,8 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
                ddz = min_img(p.z - q.z, box, hbox);
   if (ddx * ddx + ddy * ddy + ddz * ddz > half_skin2) S.rebuild = 1;  // same value from any thread
   if (l == d.topo.centre) d.centres[s] = p;
+  const int qs = d.topo.qslot[l];
+  if (qs >= 0) d.qpos[(size_t)s * d.topo.nq + qs] = p;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -643,6 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
   const double box = P.box, hbox = P.hbox, kb = P.kb, ks = P.ks, dt = P.dt;
+  const double4 cs = d.centres[s];
   const bool active = tid < np;
   const int gi = s * np + tid;
   const int lt = active ? tid : 0;
@@ -745,6 +750,12 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
         } else {
           const int qj = tb.qslot[lj];
           f = yk_term<PT>(r2, (PT)qq_s[qs * nq + qj], (PT)yk_s[qs * nq + qj], kappa, ecut_el2);
+          if (P.split) {  // uncut Yukawa: only subunit pairs within far_rf are near field
+            const double4 cb = d.centres[tag_sub(pj.w)];
+            const double cx = min_img(cs.x - cb.x, box, hbox), cy = min_img(cs.y - cb.y, box, hbox),
+                         cz = min_img(cs.z - cb.z, box, hbox);
+            if (cx * cx + cy * cy + cz * cz > P.far_rf2) f = (PT)0;
+          }
         }
         pf[e - c0] = dx * f;
         pf[kChunk + e - c0] = dy * f;
@@ -777,14 +788,20 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
       const double* q = hc + 12 * (sl >> 2) + 3 * (sl & 3);
       bx += q[0]; by += q[1]; bz += q[2];
     }
-    // src/forces.cpp:141-144: sforce + bforce + (eForce + ljForce)
-    const double elx = (double)ex + (double)lx, ely = (double)ey + (double)ly, elz = (double)ez + (double)lz;
+    // src/forces.cpp:141-144: sforce + bforce + (eForce + ljForce); in split mode eForce is
+    // the near-field list sum plus the far field of the charged bead
+    double exd = (double)ex, eyd = (double)ey, ezd = (double)ez;
+    if (d.split && tb.qslot[tid] >= 0) {
+      const double* qf = d.qfar + 3 * ((size_t)s * nq + tb.qslot[tid]);
+      exd = exd + qf[0]; eyd = eyd + qf[1]; ezd = ezd + qf[2];
+    }
+    const double elx = exd + (double)lx, ely = eyd + (double)ly, elz = ezd + (double)lz;
     double Fx = sx + bx + elx, Fy = sy + by + ely, Fz = sz + bz + elz;
     if (MODE == MODE_FORCES) {
       if (d.out_f) { d.out_f[3 * (size_t)gi] = Fx; d.out_f[3 * (size_t)gi + 1] = Fy; d.out_f[3 * (size_t)gi + 2] = Fz; }
       if (d.out_s) { d.out_s[3 * (size_t)gi] = sx; d.out_s[3 * (size_t)gi + 1] = sy; d.out_s[3 * (size_t)gi + 2] = sz; }
       if (d.out_b) { d.out_b[3 * (size_t)gi] = bx; d.out_b[3 * (size_t)gi + 1] = by; d.out_b[3 * (size_t)gi + 2] = bz; }
-      if (d.out_e) { d.out_e[3 * (size_t)gi] = ex; d.out_e[3 * (size_t)gi + 1] = ey; d.out_e[3 * (size_t)gi + 2] = ez; }
+      if (d.out_e) { d.out_e[3 * (size_t)gi] = exd; d.out_e[3 * (size_t)gi + 1] = eyd; d.out_e[3 * (size_t)gi + 2] = ezd; }
       if (d.out_lj) { d.out_lj[3 * (size_t)gi] = lx; d.out_lj[3 * (size_t)gi + 1] = ly; d.out_lj[3 * (size_t)gi + 2] = lz; }
       if (init) { d.fx[gi] = Fx; d.fy[gi] = Fy; d.fz[gi] = Fz; }
     } else if (MODE == MODE_MD) {
@@ -947,6 +964,75 @@ __global__ void k_clear_com(Dev d) {
 
 // Energies (src/md.cpp:553-593): per bead se, be, ne, ce, ke. Pair energies go to the lower
 // bead id of each pair (src/energies.cpp:31-92), using the current lists.
+// ------------------------------------------------------------------------------------------
+// Far field of the uncut Yukawa (split mode): for subunit s, the Yukawa force on each of its
+// charged beads from the charged beads of every subunit t of its replica whose centre bead
+// lies beyond far_rf (minimum image of the centre beads), pair terms as src/forces.cpp:104-107
+// with minimum-image bead vectors. Threads take subunits t in a fixed stride, then a
+// fixed-shape reduction sums them: deterministic. With `energy`, also the unshifted pair
+// energies of pairs with t > s (the reference gives a pair's energy to its lower id,
+// src/energies.cpp:24-58).
+constexpr int kFarQ = 8;         // charged beads per template the far field supports
+constexpr int kFarThreads = 128;
+
+__global__ __launch_bounds__(kFarThreads) void k_far_exact(Dev d, int energy) {
+  __shared__ double red[kFarThreads / 64][kFarQ * 4];
+  __shared__ double qq_s[kFarQ * kFarQ], yk_s[kFarQ * kFarQ];
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const int r = d.sub_rep[s];
+  const RepParams& P = d.rp[r];
+  const int nq = d.topo.nq;
+  const double box = P.box, hbox = P.hbox, kappa = P.kappa;
+  for (int k = tid; k < nq * nq; k += kFarThreads) { qq_s[k] = P.qq[k]; yk_s[k] = P.yk[k]; }
+  const double4 ca = d.centres[s];
+  double4 qa[kFarQ];
+#pragma unroll
+  for (int a = 0; a < kFarQ; a++) qa[a] = a < nq ? d.qpos[(size_t)s * nq + a] : make_double4(0, 0, 0, 0);
+  double f[kFarQ][3], e[kFarQ];
+#pragma unroll
+  for (int a = 0; a < kFarQ; a++) { f[a][0] = f[a][1] = f[a][2] = 0.0; e[a] = 0.0; }
+  __syncthreads();
+  for (int t = P.s0 + tid; t < P.s0 + P.ns; t += kFarThreads) {
+    const double4 cb = d.centres[t];
+    const double cx = min_img(ca.x - cb.x, box, hbox), cy = min_img(ca.y - cb.y, box, hbox),
+                 cz = min_img(ca.z - cb.z, box, hbox);
+    if (t == s || !(cx * cx + cy * cy + cz * cz > P.far_rf2)) continue;
+    const bool count_e = energy && t > s;
+    for (int b = 0; b < nq; b++) {
+      const double4 pb = d.qpos[(size_t)t * nq + b];
+#pragma unroll
+      for (int a = 0; a < kFarQ; a++) {
+        if (a >= nq) break;
+        const double dx = min_img(qa[a].x - pb.x, box, hbox), dy = min_img(qa[a].y - pb.y, box, hbox),
+                     dz = min_img(qa[a].z - pb.z, box, hbox);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double rr = sqrt(r2);
+        const double qq = qq_s[a * nq + b], yk = yk_s[a * nq + b];
+        const double ex = exp(-kappa * rr);
+        const double c = (qq * ex / r2) * (kappa + 1 / rr) * yk;
+        f[a][0] += dx * c; f[a][1] += dy * c; f[a][2] += dz * c;
+        if (count_e) e[a] += yk * ((qq * ex) / rr);
+      }
+    }
+  }
+  // fixed-shape reduction: wave sums, then the two waves in order
+#pragma unroll
+  for (int a = 0; a < kFarQ; a++) {
+    if (a >= nq) break;
+    const double w0 = wave_sum(f[a][0]), w1 = wave_sum(f[a][1]), w2 = wave_sum(f[a][2]), w3 = wave_sum(e[a]);
+    if ((tid & 63) == 0) {
+      red[tid >> 6][4 * a] = w0; red[tid >> 6][4 * a + 1] = w1; red[tid >> 6][4 * a + 2] = w2;
+      red[tid >> 6][4 * a + 3] = w3;
+    }
+  }
+  __syncthreads();
+  if (tid < nq * 3) {
+    const int a = tid / 3, c = tid - 3 * a;
+    d.qfar[3 * ((size_t)s * nq + a) + c] = red[0][4 * a + c] + red[1][4 * a + c];
+  }
+  if (energy && tid < nq) d.efar[(size_t)s * nq + tid] = red[0][4 * tid + 3] + red[1][4 * tid + 3];
+}
+
 __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
   extern __shared__ double smem[];
   __shared__ double red[2][5];
@@ -957,6 +1043,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
   const int r = d.sub_rep[s];
   const RepParams& P = d.rp[r];
   const double box = P.box, hbox = P.hbox;
+  const double4 cs = d.centres[s];
   const bool active = tid < np;
   const int gi = s * np + tid;
   if (active) own[tid] = d.pos[gi];
@@ -1019,11 +1106,17 @@ __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
         const double dx = min_img(pi.x - pj.x, box, hbox), dy = min_img(pi.y - pj.y, box, hbox),
                      dz = min_img(pi.z - pj.z, box, hbox);
         const double rr = sqrt(dx * dx + dy * dy + dz * dz);
-        if (rr < P.ecut_el) {
-          const double qq = P.qq[qi * nq + qj];
+        const double qq = P.qq[qi * nq + qj];
+        if (P.split) {  // uncut: unshifted (src/energies.cpp:54-58 with ecut_el = 0), near field only
+          const double4 cb = d.centres[tag_sub(pj.w)];
+          const double cx = min_img(cs.x - cb.x, box, hbox), cy = min_img(cs.y - cb.y, box, hbox),
+                       cz = min_img(cs.z - cb.z, box, hbox);
+          if (cx * cx + cy * cy + cz * cz <= P.far_rf2) e[3] += P.yk[qi * nq + qj] * ((qq * exp(-P.kappa * rr)) / rr);
+        } else if (rr < P.ecut_el) {
           e[3] += P.yk[qi * nq + qj] * ((qq * exp(-P.kappa * rr)) / rr - (qq * exp(-P.kappa * P.ecut_el)) / P.ecut_el);
         }
       }
+      if (P.split) e[3] += d.efar[gq];
     }
     const double m = d.topo.mass[tid];
     const double vx = d.vx[gi], vy = d.vy[gi], vz = d.vz[gi];
@@ -1097,6 +1190,12 @@ int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st) {
   else if (d.tile == 512) launch_force_tile<512>(d, mode, init, fp32, sm, st);
   else if (d.tile == 256) launch_force_tile<256>(d, mode, init, fp32, sm, st);
   else { set_error("unsupported k_force tile"); return SF_E_ARG; }
+  return check_launch();
+}
+
+int launch_far_exact(const Dev& d, int energy, hipStream_t st) {
+  if (d.topo.nq > kFarQ) { set_error("the far field supports at most 8 charged beads per template"); return SF_E_ARG; }
+  hipLaunchKernelGGL(k_far_exact, dim3(d.Ns), dim3(kFarThreads), 0, st, d, energy);
   return check_launch();
 }
 

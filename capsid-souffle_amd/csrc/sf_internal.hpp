@@ -24,6 +24,7 @@ struct sf_template {
   // ---- derived once per template
   int centre = 0;                 // bead closest to the template centroid
   double r_all = 0, r_q = 0;      // max distance from the centre bead: all / charged beads
+  double diam = 0;                // largest bead-bead distance
   std::vector<int> hinges;        // [nh][4] B0, B1, across(F0), across(F1)
   std::vector<int> e_off, e_other;  // per-bead incident edges (CSR, ascending edge id)
   std::vector<double> e_len0;
@@ -100,7 +101,8 @@ struct RepParams {
   double Qb[kMaxChain];          // bath masses
   double dof[kMaxChain];         // bath degrees of freedom (as double)
   int s0, ns, b0, nb, q0, nqr;   // subunit / bead / charged-slot ranges
-  int q_chain, brownian, pair_fp32, pad;
+  int q_chain, brownian, pair_fp32, split;  // split: uncut Yukawa as near (lists) + far field
+  double far_rf2;                // far field: centre-bead distance^2 beyond this
   uint32_t seed, mult6n;         // engine seed; 16807^(6 nb) mod (2^31-1)
   const double* qq;              // [nq][nq] q_i q_j lb
   const double* yk;              // [nq][nq] Yukawa size factor exp(k shc)/((1+k s_i/2)(1+k s_j/2))
@@ -142,8 +144,12 @@ struct Dev {
   double* epart;       // [Ns][5] energy partial sums
   double* ebead;       // [Nb][5] per-bead energies
   double* out_f; double* out_s; double* out_b; double* out_e; double* out_lj;  // [Nb][3], nullable
+  double4* qpos;       // [Nq] charged-bead positions (subunit-major, charged-slot order)
+  double* qfar;        // [Nq][3] far-field Yukawa force on each charged bead (split mode)
+  double* efar;        // [Nq] far-field Yukawa energy of each charged bead, pairs to higher ids
   int lj_cap, q_cap;
   int Nb, Ns, Nq, R;
+  int split;           // 1: every replica uses the uncut near/far Yukawa split
   int tile;            // list entries k_force stages in LDS per gather round (128, 256, 512)
 };
 
@@ -156,6 +162,7 @@ int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st);
 int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st);  // uses d.tile
 bool tile_supported(int tile);
 int launch_finalize(const Dev& d, int init, hipStream_t st);
+int launch_far_exact(const Dev& d, int energy, hipStream_t st);
 int launch_apply_com(const Dev& d, hipStream_t st);
 int launch_energy(const Dev& d, hipStream_t st);
 int launch_bd_uniforms_state(uint32_t state, int nb, double* out_dev, const Dev& d, hipStream_t st);

@@ -116,6 +116,11 @@ bool derive_template(sf_template& t) {
     t.r_all = std::max(t.r_all, d);
     if (t.q[l] != 0) t.r_q = std::max(t.r_q, d);
   }
+  t.diam = 0;
+  for (int a = 0; a < np; a++)
+    for (int b = a + 1; b < np; b++)
+      t.diam = std::max(t.diam, std::sqrt((t.x[a] - t.x[b]) * (t.x[a] - t.x[b]) + (t.y[a] - t.y[b]) * (t.y[a] - t.y[b]) +
+                                          (t.z[a] - t.z[b]) * (t.z[a] - t.z[b])));
   return true;
 }
 
@@ -189,6 +194,14 @@ int sf_template_info(const sf_template* t, int* n_beads, int* n_edges, int* n_fa
   if (centre_bead) *centre_bead = t->centre;
   if (si_sigma) *si_sigma = t->si_sigma;
   if (si_mass) *si_mass = t->si_mass;
+  return SF_OK;
+}
+
+int sf_template_extent(const sf_template* t, double* diameter, double* r_all, double* r_charged) {
+  if (!t) { set_error("null template"); return SF_E_ARG; }
+  if (diameter) *diameter = t->diam;
+  if (r_all) *r_all = t->r_all;
+  if (r_charged) *r_charged = t->r_q;
   return SF_OK;
 }
 

@@ -28,7 +28,7 @@ class SfParams(ctypes.Structure):
                 ("kb", ctypes.c_double), ("elj_att", ctypes.c_double), ("T", ctypes.c_double),
                 ("dt", ctypes.c_double), ("damp", ctypes.c_double), ("Q", ctypes.c_double),
                 ("q_chain", ctypes.c_int), ("brownian", ctypes.c_int), ("skin", ctypes.c_double),
-                ("pair_fp32", ctypes.c_int), ("bd_seed", ctypes.c_uint32)]
+                ("pair_fp32", ctypes.c_int), ("bd_seed", ctypes.c_uint32), ("far_rf", ctypes.c_double)]
 
 
 class SfStats(ctypes.Structure):
@@ -60,6 +60,7 @@ SIGNATURES = {
     "sf_template_free": (None, [_P]),
     "sf_template_info": (ctypes.c_int, [_P, _I, _I, _I, _I, _I, _I, _D, _D]),
     "sf_template_hinges": (ctypes.c_int, [_P, _I, ctypes.c_int]),
+    "sf_template_extent": (ctypes.c_int, [_P, _D, _D, _D]),
     "sf_params_derive": (ctypes.c_int, [_P, ctypes.POINTER(SfCli), ctypes.POINTER(SfParams)]),
     "sf_make_state": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _D, _D]),
     "sf_create": (_P, [_P, ctypes.c_int, _I, ctypes.POINTER(SfParams)]),
@@ -128,6 +129,9 @@ class Template:
         (self.n_beads, self.n_edges, self.n_faces, self.n_hinges, self.n_charged,
          self.centre) = [x.value for x in ints]
         self.si_sigma, self.si_mass = [x.value for x in dbl]
+        ext = [ctypes.c_double() for _ in range(3)]
+        _err(lib.sf_template_extent(self.ptr, *[ctypes.byref(x) for x in ext]), "sf_template_extent")
+        self.diameter, self.r_all, self.r_charged = [x.value for x in ext]
 
     def hinges(self):
         out = np.zeros((self.n_hinges, 4), dtype=np.int32)

@@ -59,7 +59,8 @@ typedef struct {
   double ecut;     /* LJ cutoff */
   double lb;       /* Bjerrum length */
   double kappa;    /* inverse screening length */
-  double ecut_el;  /* Yukawa cutoff; 0 = none (all charged pairs) */
+  double ecut_el;  /* Yukawa cutoff; 0 = none: every charged pair interacts (minimum image), as
+                      the reference's -e 0 does with a list radius that covers the box */
   double ks, kb, elj_att;
   double T;        /* reduced temperature K/298 */
   double dt, damp;
@@ -71,6 +72,9 @@ typedef struct {
   int pair_fp32;   /* 1 = LJ and Yukawa pair arithmetic in FP32 (positions, velocities,
                       thermostat and energy sums stay FP64) */
   uint32_t bd_seed;/* seed of the BD noise engine; the reference uses 69521 */
+  double far_rf;   /* uncut Yukawa only (ecut_el = 0): centre-bead distance beyond which a
+                      subunit pair is far field (its Yukawa sum is taken subunit pair by subunit
+                      pair, exactly or by a far-field model); 0 = 2.5 capsomere diameters */
 } sf_params;
 
 typedef struct {
@@ -102,11 +106,15 @@ int sf_template_info(const sf_template* t, int* n_beads, int* n_edges, int* n_fa
                      int* n_hinges, int* n_charged, int* centre_bead, double* si_sigma,
                      double* si_mass);
 
+/* Capsomere diameter (largest bead-bead distance of the template, sigma) and the largest
+ * distance of any bead / any charged bead from the centre bead. */
+int sf_template_extent(const sf_template* t, double* diameter, double* r_all, double* r_charged);
+
 /* Template-local hinge table [n_hinges][4] = (B0, B1, across F0, across F1). */
 int sf_template_hinges(const sf_template* t, int* out, int cap);
 
 /* Derives box, cutoffs, Bjerrum length and kappa from the reference's flags exactly as
- * src/md.cpp:135-173 does; skin = 0.3, pair_fp32 = 0, bd_seed = 69521. */
+ * src/md.cpp:135-173 does; skin = 0.3, pair_fp32 = 0, bd_seed = 69521, far_rf = 0. */
 int sf_params_derive(const sf_template* t, const sf_cli* cli, sf_params* out);
 
 /* Seeded initial state: non-overlapping subunits (bead gaps >= 1 sigma) at uniform random

@@ -123,6 +123,60 @@ def test_yukawa(tmp_path, r):
     assert abs(pb[1, 3] - eu) <= 1e-12 * max(1.0, abs(eu))  # bead i = global 1 < j = global 2
 
 
+def _yukawa_all_pairs(pos, q, sig, box, kappa, lb):
+    """Closed-form uncut Yukawa over every charged pair i != j, single-shift minimum image:
+    forces [N,3] and the total (unshifted) energy."""
+    n = len(pos)
+    f = np.zeros((n, 3))
+    e = 0.0
+    for i in range(n):
+        for j in range(n):
+            if i == j or q[i] == 0 or q[j] == 0:
+                continue
+            r = pos[i] - pos[j]
+            r = np.where(r > box / 2, r - box, np.where(r < -box / 2, r + box, r))
+            d = np.linalg.norm(r)
+            shc = (sig[i] + sig[j]) / 2
+            Y = math.exp(kappa * shc) / ((1 + kappa * sig[i] / 2) * (1 + kappa * sig[j] / 2))
+            U = lambda x: Y * q[i] * q[j] * lb * math.exp(-kappa * x) / x
+            f[i] += Y * q[i] * q[j] * lb * math.exp(-kappa * d) / d ** 2 * (kappa + 1 / d) * r
+            if i < j:
+                e += U(d)
+    return f, e
+
+
+@pytest.mark.parametrize("c_mM", [10.0, 100.0])
+def test_uncut_yukawa_all_pairs(tmp_path, c_mM):
+    """-e 0 with a list radius covering the box: every charged pair interacts (minimum image,
+    intra-subunit pairs included), energies unshifted (src/forces.cpp:104-107,
+    src/energies.cpp:54-58)."""
+    beads = [(0, 0, 0, 1, 0.0, 1.0, 1.0), (1.1, 0, 0, 1, 1.0, 1.0, 1.0), (0, 1.2, 0.3, 1, -2.0, 1.3, 1.0),
+             (-0.9, -0.4, 0.5, 1, 1.0, 1.1, 1.0)]
+    path = write_template(str(tmp_path / "q3"), beads=beads)
+    ns = 6
+    o = Oracle(ns, template=path, C_uM=50.0, c_mM=c_mM, ecut_c=0.0, L=80.0, B=1)
+    assert o.ecut_el == 0.0
+    rng = np.random.default_rng(5)
+    body = np.array([b[:3] for b in beads], dtype=float)
+    cen = []
+    while len(cen) < ns:
+        c = rng.uniform(-o.box / 2 + 2, o.box / 2 - 2, 3)
+        if all(np.linalg.norm(c - x) > 6.0 for x in cen):
+            cen.append(c)
+    pos = np.concatenate([body + c for c in cen])
+    o.set_state(pos)
+    fy = o.forces()[3]
+    q = np.tile([b[4] for b in beads], ns)
+    sig = np.tile([b[5] for b in beads], ns)
+    f_ref, e_ref = _yukawa_all_pairs(pos, q, sig, o.box, o.kappa, o.lb)
+    assert rel_norm(fy, f_ref) < 1e-12
+    assert abs(o.energies()[4] - e_ref) <= 1e-12 * abs(e_ref)
+    # the default list radius (ecut + 7.5) leaves the far pairs out, as the reference does
+    o2 = Oracle(ns, template=path, C_uM=50.0, c_mM=c_mM, ecut_c=0.0, L=7.5, B=1)
+    o2.set_state(pos)
+    assert rel_norm(o2.forces()[3], f_ref) > 1e-9
+
+
 @pytest.mark.parametrize("r", [0.95, 1.5, 2.4])
 def test_lj_energy_shifted(tmp_path, r):
     # attractive energy uses s = (s_i + s_j)/2 and the shift at ecut (src/energies.cpp:63-86)
