@@ -10,7 +10,7 @@
 #include <type_traits>
 #include <vector>
 
-#include "sf_internal.hpp"
+#include "sf_far.hpp"
 
 using namespace sf;
 
@@ -43,10 +43,20 @@ struct sf_handle {
   Dev d{};
   double margin = kBoundMargin;
   std::vector<double> far_rf;  // per replica, split mode
+  // far-field model (split mode): device copies of its spec and packed weights
+  FarDev fdev{};
+  bool far_model = false;      // a model is attached and has not failed an audit
+  bool far_drop = false;       // far field left out (near field only)
+  bool far_fell_back = false;
+  double far_tol = 1e-3;
+  long long far_audits = 0;
+  double far_last = 0, far_max = 0;
+  double* qfar_exact = nullptr;  // [Nq][3] exact far field at audits
+  double* audit_out = nullptr;   // [2]
   int mode = MODE_MD;
   int fp32 = 0;
   bool timing = false;
-  double t_int = 0, t_rb = 0, t_force = 0, t_red = 0, t_steps = 0;
+  double t_int = 0, t_rb = 0, t_force = 0, t_red = 0, t_steps = 0, t_far = 0;
   // snapshot for replaying a chunk
   double4* s_pos = nullptr; double* s_v = nullptr; double* s_f = nullptr; double* s_g = nullptr;
   RepState* s_rs = nullptr;
@@ -129,10 +139,33 @@ struct sf_handle {
     return SF_OK;
   }
 
-  // far field of the uncut Yukawa (split mode only)
+  // far field of the uncut Yukawa (split mode only): the model when one is attached (energies
+  // always use the exact far field)
   int launch_far(int energy) {
     if (!d.split) return SF_OK;
+    if (far_drop && !energy) return launch_fill(d.qfar, 3 * (size_t)d.Nq, 0.0, st);
+    if (far_model && !energy) {
+      if (int rc = launch_frames(d, fdev, st)) return rc;
+      return launch_far_mlp(d, fdev, st);
+    }
     return launch_far_exact(d, energy, st);
+  }
+
+  // relative error of the model's far field (in d.qfar after the last force evaluation)
+  // against the exact one at the current positions, over the force of all beads
+  int audit(double* err) {
+    Dev dx = d;
+    dx.qfar = qfar_exact;
+    if (int rc = launch_far_exact(dx, 0, st)) return rc;
+    if (int rc = launch_far_audit(d, qfar_exact, audit_out, st)) return rc;
+    double h2[2];
+    HIPCHK(hipMemcpyAsync(h2, audit_out, sizeof(h2), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *err = std::sqrt(h2[0] / std::max(h2[1], 1e-300));
+    far_audits++;
+    far_last = *err;
+    far_max = std::max(far_max, *err);
+    return SF_OK;
   }
 
   int upload_rp() {
@@ -241,21 +274,23 @@ struct sf_handle {
   }
 
   // n steps of the loop body: integrate, (conditional) list build, forces, reductions. With
-  // timing on, every kTimeEvery-th step's kernels are bracketed by events (5 per sample).
+  // timing on, every kTimeEvery-th step's kernels are bracketed by events (kEv per sample).
+  static constexpr int kEv = 6;
   int run_chunk(int n, bool tm_on) {
     for (int k = 0; k < n; k++) {
       const bool tm = tm_on && (k % kTimeEvery == 0);
-      hipEvent_t* e = tm ? &events[5 * (k / kTimeEvery)] : nullptr;
+      hipEvent_t* e = tm ? &events[kEv * (k / kTimeEvery)] : nullptr;
       if (tm) HIPCHK(hipEventRecord(e[0], st));
       if (int rc = launch_integrate(d, mode, st)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[1], st));
       if (int rc = launch_rebuild(d, 0, st)) return rc;
-      if (int rc = launch_far(0)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[2], st));
-      if (int rc = launch_force(d, mode, 0, fp32, st)) return rc;
+      if (int rc = launch_far(0)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[3], st));
-      if (int rc = launch_finalize(d, 0, st)) return rc;
+      if (int rc = launch_force(d, mode, 0, fp32, st)) return rc;
       if (tm) HIPCHK(hipEventRecord(e[4], st));
+      if (int rc = launch_finalize(d, 0, st)) return rc;
+      if (tm) HIPCHK(hipEventRecord(e[5], st));
     }
     return SF_OK;
   }
@@ -268,6 +303,7 @@ struct sf_handle {
   const void* g_key[2] = {nullptr, nullptr};
   int g_cap[2] = {0, 0};
   int g_tile = 0;
+  bool g_far = false, g_drop = false;
   std::vector<hipEvent_t> events;
 
   void drop_graph() {
@@ -275,7 +311,7 @@ struct sf_handle {
     gexec = nullptr;
   }
   int ensure_events() {
-    const size_t need = 5 * ((kChunk + kTimeEvery - 1) / kTimeEvery);
+    const size_t need = kEv * ((kChunk + kTimeEvery - 1) / kTimeEvery);
     while (events.size() < need) {
       hipEvent_t e;
       HIPCHK(hipEventCreate(&e));
@@ -292,7 +328,8 @@ struct sf_handle {
     // the captured kernel arguments hold the list buffers and their capacities; a regrown
     // buffer may come back at the same address, so the capacities are part of the key
     if (!gexec || g_n != n || g_tm != timing || g_key[0] != d.lj_list || g_key[1] != d.q_list ||
-        g_cap[0] != d.lj_cap || g_cap[1] != d.q_cap || g_tile != d.tile) {
+        g_cap[0] != d.lj_cap || g_cap[1] != d.q_cap || g_tile != d.tile || g_far != far_model ||
+        g_drop != far_drop) {
       drop_graph();
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -304,7 +341,7 @@ struct sf_handle {
       (void)hipGraphDestroy(g);
       if (ei != hipSuccess) { gexec = nullptr; set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return SF_E_HIP; }
       g_n = n; g_tm = timing; g_key[0] = d.lj_list; g_key[1] = d.q_list;
-      g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap; g_tile = d.tile;
+      g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap; g_tile = d.tile; g_far = far_model; g_drop = far_drop;
     }
     HIPCHK(hipGraphLaunch(gexec, st));
     return SF_OK;
@@ -372,6 +409,7 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
   // template tables
   TopoDev& td = d.topo;
   td.np = np; td.nh = t->nh(); td.nq = nq; td.centre = t->centre;
+  td.fp1 = t->fp1; td.fp2 = t->fp2;
   auto up = [&](auto** dst, const auto& v) -> int {
     typedef typename std::remove_reference<decltype(v[0])>::type T0;
     typedef typename std::remove_const<T0>::type T;
@@ -617,7 +655,7 @@ int sf_energies(sf_handle* h, double* totals, double* per_bead) {
 
 int sf_step(sf_handle* h, int n_steps) {
   if (!h || n_steps < 0) { set_error("bad arguments to sf_step"); return SF_E_ARG; }
-  double ti = 0, trb = 0, tf = 0, tr = 0, nt = 0;
+  double ti = 0, trb = 0, tf = 0, tr = 0, nt = 0, tfar = 0;
   int done = 0, rc = SF_OK;
   int retries = 0;
   while (done < n_steps) {
@@ -635,23 +673,30 @@ int sf_step(sf_handle* h, int n_steps) {
     }
     if (h->timing) {
       for (int k = 0; k < n; k += kTimeEvery) {
-        const hipEvent_t* e = &h->events[5 * (k / kTimeEvery)];
-        float a = 0, b = 0, c = 0, x = 0;
+        const hipEvent_t* e = &h->events[sf_handle::kEv * (k / kTimeEvery)];
+        float a = 0, b = 0, fa = 0, c = 0, x = 0;
         (void)hipEventElapsedTime(&a, e[0], e[1]);
         (void)hipEventElapsedTime(&b, e[1], e[2]);
-        (void)hipEventElapsedTime(&c, e[2], e[3]);
-        (void)hipEventElapsedTime(&x, e[3], e[4]);
-        ti += a; trb += b; tf += c; tr += x;
+        (void)hipEventElapsedTime(&fa, e[2], e[3]);
+        (void)hipEventElapsedTime(&c, e[3], e[4]);
+        (void)hipEventElapsedTime(&x, e[4], e[5]);
+        ti += a; trb += b; tfar += fa; tf += c; tr += x;
         nt += 1;
       }
     }
     done += n;
+    if (h->far_model) {  // audit the far-field model after every chunk
+      double err = 0;
+      if ((rc = h->audit(&err))) break;
+      if (!(err <= h->far_tol)) { h->far_model = false; h->far_fell_back = true; }
+    }
   }
   if (rc) return rc;
   if ((rc = launch_apply_com(h->d, h->st))) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
   if (h->timing && nt > 0) {
     h->t_int = ti / nt; h->t_rb = trb / nt; h->t_force = tf / nt; h->t_red = tr / nt; h->t_steps = nt;
+    h->t_far = tfar / nt;
   }
   return SF_OK;
 }
@@ -673,6 +718,7 @@ int sf_get_stats(sf_handle* h, sf_stats* o) {
   o->n_beads = h->Nb; o->n_subunits = h->Ns; o->n_charged = h->Nq; o->n_replicas = h->R;
   o->k_integrate_ms = h->t_int; o->k_rebuild_ms = h->t_rb; o->k_force_ms = h->t_force; o->k_reduce_ms = h->t_red;
   o->k_timed_steps = h->t_steps;
+  o->k_far_ms = h->t_far;
   o->vcom = s[0].vcom;
   return SF_OK;
 }
@@ -775,7 +821,7 @@ int sf_autotune(sf_handle* h, const sf_tune_opts* opt, sf_tune_result* res, doub
     return h->request_rebuild();
   };
   const bool timing = h->timing;
-  const double t_save[5] = {h->t_int, h->t_rb, h->t_force, h->t_red, h->t_steps};
+  const double t_save[6] = {h->t_int, h->t_rb, h->t_force, h->t_red, h->t_steps, h->t_far};
   h->timing = false;
   int rc = copy_state(true);
   double best = 1e300;
@@ -796,12 +842,72 @@ int sf_autotune(sf_handle* h, const sf_tune_opts* opt, sf_tune_result* res, doub
   if (!rc) rc = apply(skins[bi], tiles[bj]);
   h->timing = timing;
   h->t_int = t_save[0]; h->t_rb = t_save[1]; h->t_force = t_save[2]; h->t_red = t_save[3]; h->t_steps = t_save[4];
+  h->t_far = t_save[5];
   h->release(a_pos); h->release(a_v); h->release(a_f); h->release(a_g); h->release(a_rs);
   if (rc) return rc;
   res->skin = skins[bi];
   res->tile = tiles[bj];
   res->ms_per_step = best;
   res->n_trials = n;
+  return SF_OK;
+}
+
+int sf_set_far_model(sf_handle* h, const sf_far_model* m, double tol) {
+  if (!h) { set_error("null handle"); return SF_E_ARG; }
+  if (!h->d.split) { set_error("a far-field model needs the uncut Yukawa (ecut_el = 0)"); return SF_E_ARG; }
+  if (!m) { h->far_model = false; return SF_OK; }
+  if (h->mode != MODE_MD) { set_error("the far-field model is audited against MD forces; use MD"); return SF_E_ARG; }
+  if (!(tol > 0)) { set_error("tolerance must be positive"); return SF_E_ARG; }
+  FarSpec spec;
+  const unsigned short* wsrc;
+  const float* bsrc;
+  far_model_device(m, &spec, &wsrc, &bsrc);
+  if (spec.nq != h->t.nq()) { set_error("model and handle templates differ"); return SF_E_ARG; }
+  for (int r = 0; r < h->R; r++) {
+    if (h->far_rf[r] < spec.rf - 1e-9) { set_error("handle far_rf is below the model's split radius"); return SF_E_ARG; }
+    const double k = h->prm[r].kappa;
+    if (k < spec.kmin * (1 - 1e-9) || k > spec.kmax * (1 + 1e-9)) {
+      set_error("a replica's salt is outside the model's training range"); return SF_E_ARG;
+    }
+  }
+  if (!h->fdev.spec) {
+    FarSpec* ds; unsigned short* dw; float* db; float* fr;
+    if (h->alloc(&ds, 1) || h->alloc(&dw, kFPackN) || h->alloc(&db, 3 * kFHid + kFOutPad) ||
+        h->alloc(&fr, (size_t)h->Ns * 12) || h->alloc(&h->qfar_exact, 3 * (size_t)std::max(h->Nq, 1)) ||
+        h->alloc(&h->audit_out, 2))
+      return SF_E_HIP;
+    h->fdev.spec = ds; h->fdev.wpack = dw; h->fdev.bias = db; h->fdev.frames = fr;
+  }
+  HIPCHK(hipMemcpyAsync((void*)h->fdev.spec, &spec, sizeof(FarSpec), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync((void*)h->fdev.wpack, wsrc, kFPackN * sizeof(unsigned short), hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipMemcpyAsync((void*)h->fdev.bias, bsrc, (3 * kFHid + kFOutPad) * sizeof(float), hipMemcpyDeviceToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->far_tol = tol;
+  h->far_model = true;
+  h->far_fell_back = false;
+  h->far_audits = 0; h->far_last = 0; h->far_max = 0;
+  return SF_OK;
+}
+
+int sf_far_drop(sf_handle* h, int drop) {
+  if (!h) { set_error("null handle"); return SF_E_ARG; }
+  if (!h->d.split) { set_error("no far field: the Yukawa is cut"); return SF_E_ARG; }
+  h->far_drop = drop != 0;
+  return SF_OK;
+}
+
+int sf_far_audit(sf_handle* h, double* rel_err) {
+  if (!h || !rel_err) { set_error("null argument"); return SF_E_ARG; }
+  if (!h->far_model) { set_error("no far-field model in use"); return SF_E_ARG; }
+  // the model's far field and the integrator force at the current positions
+  if (int rc = h->launch_far(0)) return rc;
+  return h->audit(rel_err);
+}
+
+int sf_get_far_stats(sf_handle* h, sf_far_stats* o) {
+  if (!h || !o) { set_error("null argument"); return SF_E_ARG; }
+  o->audits = h->far_audits; o->last_err = h->far_last; o->max_err = h->far_max;
+  o->using_model = h->far_model ? 1 : 0; o->fell_back = h->far_fell_back ? 1 : 0;
   return SF_OK;
 }
 

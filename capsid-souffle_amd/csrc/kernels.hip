@@ -1193,6 +1193,17 @@ int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st) {
   return check_launch();
 }
 
+__global__ void k_fill(double* p, size_t n, double v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+int launch_fill(double* p, size_t n, double v, hipStream_t st) {
+  if (n == 0) return SF_OK;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  return check_launch();
+}
+
 int launch_far_exact(const Dev& d, int energy, hipStream_t st) {
   if (d.topo.nq > kFarQ) { set_error("the far field supports at most 8 charged beads per template"); return SF_E_ARG; }
   hipLaunchKernelGGL(k_far_exact, dim3(d.Ns), dim3(kFarThreads), 0, st, d, energy);

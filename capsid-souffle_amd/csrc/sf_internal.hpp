@@ -25,6 +25,11 @@ struct sf_template {
   int centre = 0;                 // bead closest to the template centroid
   double r_all = 0, r_q = 0;      // max distance from the centre bead: all / charged beads
   double diam = 0;                // largest bead-bead distance
+  // body frame of a subunit: e1 along (x_fp1 - x_centre), e2 the part of (x_fp2 - x_centre)
+  // normal to e1, e3 = e1 x e2; qbody = charged beads' template coordinates in that frame
+  int fp1 = 0, fp2 = 0;
+  bool frame_ok = false;          // fp2 is off the e1 axis
+  std::vector<double> qbody;      // [nq][3]
   std::vector<int> hinges;        // [nh][4] B0, B1, across(F0), across(F1)
   std::vector<int> e_off, e_other;  // per-bead incident edges (CSR, ascending edge id)
   std::vector<double> e_len0;
@@ -49,6 +54,20 @@ constexpr uint32_t kLcgMul = 16807u;
 
 void set_error(const std::string& s);
 
+// Body frame from u1 = x_fp1 - x_centre and u2 = x_fp2 - x_centre: e1 = u1/|u1|,
+// e2 = (u2 - (u2.e1) e1)/|...|, e3 = e1 x e2. R is row-major with columns e1 e2 e3, so
+// lab = R body and body = R^T lab.
+__host__ __device__ inline void body_frame(const double* u1, const double* u2, double* R) {
+  const double n1 = sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+  const double e1[3] = {u1[0] / n1, u1[1] / n1, u1[2] / n1};
+  const double a = u2[0] * e1[0] + u2[1] * e1[1] + u2[2] * e1[2];
+  double w[3] = {u2[0] - a * e1[0], u2[1] - a * e1[1], u2[2] - a * e1[2]};
+  const double nw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  const double e2[3] = {w[0] / nw, w[1] / nw, w[2] / nw};
+  const double e3[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  for (int k = 0; k < 3; k++) { R[3 * k] = e1[k]; R[3 * k + 1] = e2[k]; R[3 * k + 2] = e3[k]; }
+}
+
 // Bead tags: subunit id in the high 32 bits, template-local index in the low 32 bits,
 // carried bit-for-bit in the w component of the position double4.
 inline double make_tag(int s, int l) {
@@ -68,6 +87,7 @@ constexpr int kRbLjBuf = 24;      // per-thread LDS buffer of a bead's LJ list a
 // Device view of the template tables.
 struct TopoDev {
   int np, nh, nq, centre;
+  int fp1, fp2;                   // frame beads of the body frame
   const double* mass;
   const double* sigma;
   const int* type;
@@ -163,6 +183,7 @@ int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st);  /
 bool tile_supported(int tile);
 int launch_finalize(const Dev& d, int init, hipStream_t st);
 int launch_far_exact(const Dev& d, int energy, hipStream_t st);
+int launch_fill(double* p, size_t n, double v, hipStream_t st);  // a kernel, so graphs may hold it
 int launch_apply_com(const Dev& d, hipStream_t st);
 int launch_energy(const Dev& d, hipStream_t st);
 int launch_bd_uniforms_state(uint32_t state, int nb, double* out_dev, const Dev& d, hipStream_t st);

@@ -121,6 +121,38 @@ bool derive_template(sf_template& t) {
     for (int b = a + 1; b < np; b++)
       t.diam = std::max(t.diam, std::sqrt((t.x[a] - t.x[b]) * (t.x[a] - t.x[b]) + (t.y[a] - t.y[b]) * (t.y[a] - t.y[b]) +
                                           (t.z[a] - t.z[b]) * (t.z[a] - t.z[b])));
+  // body frame: fp1 = bead farthest from the centre bead, fp2 = bead farthest from that axis
+  // (first index on ties)
+  const int c = t.centre;
+  double e1[3] = {0, 0, 0}, b1 = -1;
+  for (int l = 0; l < np; l++) {
+    const double u[3] = {t.x[l] - t.x[c], t.y[l] - t.y[c], t.z[l] - t.z[c]};
+    const double n2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    if (n2 > b1 + 1e-12) { b1 = n2; t.fp1 = l; }
+  }
+  {
+    const double u[3] = {t.x[t.fp1] - t.x[c], t.y[t.fp1] - t.y[c], t.z[t.fp1] - t.z[c]};
+    const double n = std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (int k = 0; k < 3; k++) e1[k] = n > 0 ? u[k] / n : 0.0;
+  }
+  double b2 = -1;
+  for (int l = 0; l < np; l++) {
+    const double u[3] = {t.x[l] - t.x[c], t.y[l] - t.y[c], t.z[l] - t.z[c]};
+    const double a = u[0] * e1[0] + u[1] * e1[1] + u[2] * e1[2];
+    const double w2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2] - a * a;
+    if (w2 > b2 + 1e-12) { b2 = w2; t.fp2 = l; }
+  }
+  double R[9];
+  const double u1[3] = {t.x[t.fp1] - t.x[c], t.y[t.fp1] - t.y[c], t.z[t.fp1] - t.z[c]};
+  const double u2[3] = {t.x[t.fp2] - t.x[c], t.y[t.fp2] - t.y[c], t.z[t.fp2] - t.z[c]};
+  body_frame(u1, u2, R);
+  t.qbody.assign(3 * t.qlocal.size(), 0.0);
+  for (size_t k = 0; k < t.qlocal.size(); k++) {
+    const int l = t.qlocal[k];
+    const double u[3] = {t.x[l] - t.x[c], t.y[l] - t.y[c], t.z[l] - t.z[c]};
+    for (int a = 0; a < 3; a++) t.qbody[3 * k + a] = R[a] * u[0] + R[3 + a] * u[1] + R[6 + a] * u[2];  // R^T u
+  }
+  t.frame_ok = b2 > 1e-12;
   return true;
 }
 
@@ -202,6 +234,15 @@ int sf_template_extent(const sf_template* t, double* diameter, double* r_all, do
   if (diameter) *diameter = t->diam;
   if (r_all) *r_all = t->r_all;
   if (r_charged) *r_charged = t->r_q;
+  return SF_OK;
+}
+
+int sf_template_frame(const sf_template* t, int* fp1, int* fp2, double* qbody) {
+  if (!t) { set_error("null template"); return SF_E_ARG; }
+  if (!t->frame_ok) { set_error("template beads are collinear: no body frame"); return SF_E_ARG; }
+  if (fp1) *fp1 = t->fp1;
+  if (fp2) *fp2 = t->fp2;
+  if (qbody) for (size_t k = 0; k < t->qbody.size(); k++) qbody[k] = t->qbody[k];
   return SF_OK;
 }
 

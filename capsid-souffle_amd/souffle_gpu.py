@@ -38,7 +38,7 @@ class SfStats(ctypes.Structure):
                 ("n_replicas", ctypes.c_int32), ("k_integrate_ms", ctypes.c_double),
                 ("k_rebuild_ms", ctypes.c_double),
                 ("k_force_ms", ctypes.c_double), ("k_reduce_ms", ctypes.c_double),
-                ("k_timed_steps", ctypes.c_double), ("vcom", ctypes.c_double)]
+                ("k_timed_steps", ctypes.c_double), ("vcom", ctypes.c_double), ("k_far_ms", ctypes.c_double)]
 
 
 class SfTuneOpts(ctypes.Structure):
@@ -51,10 +51,28 @@ class SfTuneResult(ctypes.Structure):
                 ("n_trials", ctypes.c_int)]
 
 
+class SfTrainOpts(ctypes.Structure):
+    _fields_ = [("n_samples", ctypes.c_int), ("epochs", ctypes.c_int), ("batch", ctypes.c_int),
+                ("lr", ctypes.c_double), ("val_frac", ctypes.c_double), ("seed", ctypes.c_uint64),
+                ("c_lo_mM", ctypes.c_double), ("c_hi_mM", ctypes.c_double)]
+
+
+class SfTrainReport(ctypes.Structure):
+    _fields_ = [("train_loss", ctypes.c_double), ("val_loss", ctypes.c_double),
+                ("val_rel_err_force", ctypes.c_double), ("val_rel_err_torque", ctypes.c_double),
+                ("train_rel_err_force", ctypes.c_double), ("seconds", ctypes.c_double), ("steps", ctypes.c_int)]
+
+
+class SfFarStats(ctypes.Structure):
+    _fields_ = [("audits", ctypes.c_int64), ("last_err", ctypes.c_double), ("max_err", ctypes.c_double),
+                ("using_model", ctypes.c_int), ("fell_back", ctypes.c_int)]
+
+
 # every entry point of include/souffle_gpu.h: name -> (restype, argtypes)
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
+_FP = ctypes.POINTER(ctypes.c_float)
 SIGNATURES = {
     "sf_template_load": (_P, [ctypes.c_char_p]),
     "sf_template_free": (None, [_P]),
@@ -78,6 +96,20 @@ SIGNATURES = {
     "sf_get_skin": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "sf_set_tile": (ctypes.c_int, [_P, ctypes.c_int]),
     "sf_autotune": (ctypes.c_int, [_P, ctypes.POINTER(SfTuneOpts), ctypes.POINTER(SfTuneResult), _D]),
+    "sf_template_frame": (ctypes.c_int, [_P, _I, _I, _D]),
+    "sf_far_model_create": (_P, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
+    "sf_far_model_destroy": (None, [_P]),
+    "sf_far_model_info": (ctypes.c_int, [_P, _D, _D, _D, _I]),
+    "sf_far_model_params": (ctypes.c_int, [_P, _FP, _FP, _FP, _FP]),
+    "sf_far_model_samples": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                                            _FP, _FP, _D]),
+    "sf_far_model_grad": (ctypes.c_int, [_P, ctypes.c_int, _FP, _FP, _FP, _D]),
+    "sf_far_model_infer": (ctypes.c_int, [_P, ctypes.c_int, _FP, _FP, ctypes.c_int]),
+    "sf_far_model_train": (ctypes.c_int, [_P, ctypes.POINTER(SfTrainOpts), ctypes.POINTER(SfTrainReport)]),
+    "sf_set_far_model": (ctypes.c_int, [_P, _P, ctypes.c_double]),
+    "sf_far_audit": (ctypes.c_int, [_P, _D]),
+    "sf_far_drop": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sf_get_far_stats": (ctypes.c_int, [_P, ctypes.POINTER(SfFarStats)]),
     "sf_last_error": (ctypes.c_char_p, []),
 }
 
@@ -132,6 +164,13 @@ class Template:
         ext = [ctypes.c_double() for _ in range(3)]
         _err(lib.sf_template_extent(self.ptr, *[ctypes.byref(x) for x in ext]), "sf_template_extent")
         self.diameter, self.r_all, self.r_charged = [x.value for x in ext]
+
+    def frame(self):
+        """(fp1, fp2, qbody [n_charged][3]) of the subunit body frame."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        qb = np.zeros((self.n_charged, 3))
+        _err(load().sf_template_frame(self.ptr, ctypes.byref(a), ctypes.byref(b), _dp(qb)), "sf_template_frame")
+        return a.value, b.value, qb
 
     def hinges(self):
         out = np.zeros((self.n_hinges, 4), dtype=np.int32)
@@ -251,9 +290,95 @@ class System:
         _err(_lib.sf_autotune(self.ptr, ctypes.byref(o), ctypes.byref(res), _dp(ms)), "sf_autotune")
         return {k: getattr(res, k) for k, _ in SfTuneResult._fields_}, ms
 
+    def set_far_model(self, model, tol=1e-3):
+        _err(_lib.sf_set_far_model(self.ptr, model.ptr if model is not None else None, float(tol)), "sf_set_far_model")
+
+    def far_drop(self, drop=True):
+        _err(_lib.sf_far_drop(self.ptr, 1 if drop else 0), "sf_far_drop")
+
+    def far_audit(self):
+        v = ctypes.c_double()
+        _err(_lib.sf_far_audit(self.ptr, ctypes.byref(v)), "sf_far_audit")
+        return v.value
+
+    def far_stats(self):
+        s = SfFarStats()
+        _err(_lib.sf_get_far_stats(self.ptr, ctypes.byref(s)), "sf_get_far_stats")
+        return {k: getattr(s, k) for k, _ in SfFarStats._fields_}
+
     def close(self):
         if getattr(self, "ptr", None):
             _lib.sf_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_FP)
+
+
+class FarModel:
+    """Far-field network of the uncut Yukawa (13 -> 128 -> 128 -> 128 -> 6, SiLU) on the GPU."""
+
+    N_IN, N_OUT = 13, 6
+
+    def __init__(self, template, far_rf=0.0, c_lo=5.0, c_hi=200.0, seed=1):
+        lib = load()
+        self.ptr = lib.sf_far_model_create(template.ptr, float(far_rf), float(c_lo), float(c_hi), int(seed))
+        if not self.ptr:
+            raise SfError(lib.sf_last_error().decode())
+        rf, k0, k1, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        _err(lib.sf_far_model_info(self.ptr, ctypes.byref(rf), ctypes.byref(k0), ctypes.byref(k1), ctypes.byref(n)),
+             "sf_far_model_info")
+        self.far_rf, self.kappa_min, self.kappa_max, self.n_params = rf.value, k0.value, k1.value, n.value
+
+    def params(self):
+        p = np.zeros(self.n_params, dtype=np.float32)
+        norm = np.zeros(12, dtype=np.float32)
+        _err(_lib.sf_far_model_params(self.ptr, _fp(p), None, _fp(norm), None), "sf_far_model_params")
+        return p, norm[:6].copy(), norm[6:].copy()
+
+    def set_params(self, p=None, mu=None, sd=None):
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float32)
+        norm = None
+        if mu is not None:
+            norm = np.ascontiguousarray(np.concatenate([mu, sd]), dtype=np.float32)
+        _err(_lib.sf_far_model_params(self.ptr, None, _fp(p), None, _fp(norm)), "sf_far_model_params")
+
+    def samples(self, n, seed, c_lo, c_hi):
+        x = np.zeros((n, 13), dtype=np.float32)
+        y = np.zeros((n, 6), dtype=np.float32)
+        pose = np.zeros((n, 16))
+        _err(_lib.sf_far_model_samples(self.ptr, int(n), int(seed), float(c_lo), float(c_hi), _fp(x), _fp(y), _dp(pose)),
+             "sf_far_model_samples")
+        return x, y, pose
+
+    def grad(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = ctypes.c_double()
+        _err(_lib.sf_far_model_grad(self.ptr, len(x), _fp(x), _fp(y), _fp(g), ctypes.byref(loss)), "sf_far_model_grad")
+        return loss.value, g
+
+    def infer(self, x, bf16_mfma=True):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((len(x), 6), dtype=np.float32)
+        _err(_lib.sf_far_model_infer(self.ptr, len(x), _fp(x), _fp(y), 1 if bf16_mfma else 0), "sf_far_model_infer")
+        return y
+
+    def train(self, n_samples=1000000, epochs=20, batch=4096, lr=2e-3, val_frac=0.05, seed=12345, c_lo=0.0, c_hi=0.0):
+        o = SfTrainOpts(n_samples=n_samples, epochs=epochs, batch=batch, lr=lr, val_frac=val_frac, seed=seed,
+                        c_lo_mM=c_lo, c_hi_mM=c_hi)
+        r = SfTrainReport()
+        _err(_lib.sf_far_model_train(self.ptr, ctypes.byref(o), ctypes.byref(r)), "sf_far_model_train")
+        return {k: getattr(r, k) for k, _ in SfTrainReport._fields_}
+
+    def close(self):
+        if getattr(self, "ptr", None) and _lib is not None:
+            _lib.sf_far_model_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):

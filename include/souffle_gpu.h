@@ -91,6 +91,7 @@ typedef struct {
   double k_reduce_ms;
   double k_timed_steps;   /* steps the means were taken over (0 if timing was off) */
   double vcom;            /* replica 0: |centre-of-mass velocity| removed at the last step */
+  double k_far_ms;        /* far field of the uncut Yukawa (exact or model), as the means above */
 } sf_stats;
 
 /* ---- template and parameters ---------------------------------------------------------- */
@@ -109,6 +110,13 @@ int sf_template_info(const sf_template* t, int* n_beads, int* n_edges, int* n_fa
 /* Capsomere diameter (largest bead-bead distance of the template, sigma) and the largest
  * distance of any bead / any charged bead from the centre bead. */
 int sf_template_extent(const sf_template* t, double* diameter, double* r_all, double* r_charged);
+
+/* Body frame of a subunit (used by the far-field model): e1 along (frame bead 1 - centre bead),
+ * e2 the part of (frame bead 2 - centre bead) normal to e1, e3 = e1 x e2. Frame bead 1 is the
+ * bead farthest from the centre bead, frame bead 2 the bead farthest from that axis.
+ * qbody (nullable) [n_charged][3]: charged beads' template coordinates in that frame.
+ * Returns SF_E_ARG if the template's beads are collinear. */
+int sf_template_frame(const sf_template* t, int* fp1, int* fp2, double* qbody);
 
 /* Template-local hinge table [n_hinges][4] = (B0, B1, across F0, across F1). */
 int sf_template_hinges(const sf_template* t, int* out, int cap);
@@ -203,6 +211,86 @@ typedef struct {
  * step counter and the trajectory that follows are exactly as without the call.
  * opts may be NULL (defaults); trial_ms (nullable) receives [n_skins][n_tiles] ms/step. */
 int sf_autotune(sf_handle* h, const sf_tune_opts* opts, sf_tune_result* out, double* trial_ms);
+
+/* ---- far-field model of the uncut Yukawa (no reference counterpart: the reference sums every
+ * list pair, src/forces.cpp:104-107). A small network (13 -> 128 -> 128 -> 128 -> 6, SiLU)
+ * maps the pose of subunit B seen from subunit A (B's centre in A's body frame, relative
+ * orientation, kappa) to the force on A's charged beads and its torque about A's centre bead;
+ * trained on the GPU on poses scored by the exact pair sum; evaluated with BF16 MFMA and FP32
+ * accumulation for subunit pairs beyond far_rf. ----------------------------------------- */
+
+typedef struct sf_far_model sf_far_model;
+
+/* New model for a template (1..8 charged beads), split radius far_rf (0 = 2.5 capsomere
+ * diameters) and salt range [c_lo_mM, c_hi_mM]; Glorot-uniform weights from seed. */
+sf_far_model* sf_far_model_create(const sf_template* t, double far_rf, double c_lo_mM, double c_hi_mM,
+                                  uint64_t seed);
+void sf_far_model_destroy(sf_far_model* m);
+int sf_far_model_info(const sf_far_model* m, double* far_rf, double* kappa_min, double* kappa_max, int* n_params);
+
+/* Parameters (FP32, packed W1 [128][13], b1, W2 [128][128], b2, W3, b3, W4 [6][128], b4) and
+ * output standardisation (mean[6], sd[6]): any pointer may be NULL. */
+int sf_far_model_params(sf_far_model* m, float* get, const float* set, float* norm_get, const float* norm_set);
+
+/* n seeded training poses generated and scored on the GPU: features x [n][13], scaled targets
+ * y [n][6] (force / S, torque / (S rq)) with S = lb (kappa + 1/d) / d exp(-kappa d)
+ * sum_a exp(kappa p_a) sum_b exp(kappa p'_b), p_a and p'_b the charges' projections towards
+ * the other subunit; pose [n][16] = (d_vec[3], Q[9] row-major, kappa, c_mM, S, 0); any output
+ * may be NULL. */
+int sf_far_model_samples(sf_far_model* m, int n, uint64_t seed, double c_lo_mM, double c_hi_mM, float* x, float* y,
+                         double* pose);
+
+/* Mean squared error of the network on (x [n][13], standardised y [n][6]) and its gradient
+ * with respect to every parameter, by the GPU training kernels. */
+int sf_far_model_grad(sf_far_model* m, int n, const float* x, const float* y, float* grad, double* loss);
+
+/* Network outputs (standardised units) for x [n][13]: BF16 MFMA (bf16_mfma = 1, the engine's
+ * kernel) or the FP32 training forward (0). */
+int sf_far_model_infer(sf_far_model* m, int n, const float* x, float* y, int bf16_mfma);
+
+typedef struct {
+  int n_samples;       /* training poses (<= 0: 1,000,000) */
+  int epochs;          /* <= 0: 20 */
+  int batch;           /* <= 0: 4096 */
+  double lr;           /* Adam step, cosine-decayed to lr/50; <= 0: 2e-3 */
+  double val_frac;     /* held-out share; <= 0: 0.05 */
+  uint64_t seed;
+  double c_lo_mM, c_hi_mM;  /* salt range of the poses; <= 0: the model's range */
+} sf_train_opts;
+
+typedef struct {
+  double train_loss, val_loss;   /* MSE in standardised units */
+  double val_rel_err_force;      /* |F_net - F| / |F| over held-out poses (scaled units) */
+  double val_rel_err_torque;
+  double train_rel_err_force;
+  double seconds;
+  int steps;
+} sf_train_report;
+
+int sf_far_model_train(sf_far_model* m, const sf_train_opts* opts, sf_train_report* report);
+
+/* Uses model m for the far field of an uncut-Yukawa handle (NULL: exact far field again).
+ * The handle copies what it needs; m may be destroyed afterwards. After every 100-step chunk
+ * of sf_step the far field is audited against the exact one: if
+ * |F_far,model - F_far,exact| / |F| over all beads exceeds tol, the handle falls back to the
+ * exact far field for the rest of its life (sf_far_stats reports it). MD only. */
+int sf_set_far_model(sf_handle* h, const sf_far_model* m, double tol);
+
+/* drop = 1: leave the far field out (the uncut Yukawa truncated at far_rf by centre-bead
+ * distance), the method without a far-field model; 0: back to the exact or model far field. */
+int sf_far_drop(sf_handle* h, int drop);
+
+/* Audit now: relative error of the model's far field against the exact one, as above. */
+int sf_far_audit(sf_handle* h, double* rel_err);
+
+typedef struct {
+  int64_t audits;
+  double last_err, max_err;
+  int using_model;       /* 1 while the model supplies the far field */
+  int fell_back;         /* 1 if an audit exceeded the tolerance */
+} sf_far_stats;
+
+int sf_get_far_stats(sf_handle* h, sf_far_stats* out);
 
 const char* sf_last_error(void);
 
