@@ -127,9 +127,15 @@ def probe_copy(mb):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["md", "replicas"], default="md",
-                    help="md: one 2000-subunit system (default); replicas: the 256-replica sweep")
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (md 20000, replicas 3000)")
+    ap.add_argument("--workload", choices=["md", "replicas", "surrogate"], default="md",
+                    help="md: one 2000-subunit system (default); replicas: the 256-replica sweep; "
+                         "surrogate: 5000 subunits, uncut Yukawa, far-field network vs exact")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (md 20000, replicas 3000, surrogate 1000)")
+    ap.add_argument("--sur-subunits", type=int, default=5000)
+    ap.add_argument("--sur-salt", type=float, default=10.0)
+    ap.add_argument("--sur-samples", type=int, default=1000000)
+    ap.add_argument("--sur-epochs", type=int, default=20)
+    ap.add_argument("--sur-steps", type=int, default=500, help="timed steps of the surrogate summary in the md line")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--tune-steps", type=int, default=100)
@@ -144,7 +150,7 @@ def main():
     ap.add_argument("--probe-copy-mb", type=int, default=0, help="run a known-size copy kernel first")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 3000 if args.workload == "replicas" else 20000
+        args.steps = {"replicas": 3000, "surrogate": 1000}.get(args.workload, 20000)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one large system would shard "
               "into subunit-owning spatial domains with one halo exchange per step).", file=sys.stderr)
@@ -158,6 +164,8 @@ def main():
 
     if args.workload == "replicas":
         out = bench_replicas(args)
+    elif args.workload == "surrogate":
+        out = bench_surrogate(args)
     else:
         out = bench_md(args)
     print(json.dumps(out), flush=True)
@@ -261,6 +269,85 @@ def bench_replicas(args):
     return out
 
 
+def run_surrogate(args, steps):
+    """Uncut Yukawa at 5000 subunits (C = 800 uM, c = args.sur_salt mM, other parameters as the
+    default workload): the plain FP64 path (exact far field, subunit-pair tiles) against FP32 pair
+    arithmetic with the far field from the network (trained here on the GPU), from the same
+    seeded state; force error of the second against the first on one state."""
+    import numpy as np
+    import souffle_gpu as sg
+    from oracle_lib import TEMPLATE
+    t = sg.Template(TEMPLATE)
+    ns = args.sur_subunits
+    model = sg.FarModel(t, 20.0, 5.0, 50.0, seed=2)
+    t0 = time.perf_counter()
+    rep = model.train(n_samples=args.sur_samples, epochs=args.sur_epochs, batch=4096, lr=2e-3, seed=77)
+    train_s = time.perf_counter() - t0
+    cfg = dict(CFG, c_mM=args.sur_salt, ecut_c=0.0)
+    out = {"train": rep, "train_wall_s": train_s}
+    systems = {}
+    for name, fp32, use_model in (("fp64_exact", False, False), ("fp32_model", True, True)):
+        p = t.params(n_subunits=ns, **cfg)
+        p.far_rf = 20.0
+        p.skin = args.skin
+        p.pair_fp32 = 1 if fp32 else 0
+        pos, vel = t.make_state(ns, p.box, p.T, 2000)
+        s = sg.System(t, ns, p)
+        s.set_state(pos, vel)
+        if use_model:
+            s.set_far_model(model, tol=1e-3)
+        s.step(100)
+        t1 = time.perf_counter()
+        s.step(steps)
+        wall = time.perf_counter() - t1
+        s.set_timing(True)
+        s.step(min(steps, 500))
+        s.set_timing(False)
+        st = s.stats()
+        out[name] = {"bead_steps_per_s": s.n_beads * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                     "kernels_ms": {"k_integrate": st["k_integrate_ms"], "k_rebuild": st["k_rebuild_ms"],
+                                    "k_far": st["k_far_ms"], "k_force": st["k_force_ms"],
+                                    "k_finalize": st["k_reduce_ms"]}}
+        if use_model:
+            out[name]["audits"] = s.far_stats()
+        systems[name] = s
+    out["speedup_vs_fp64_exact"] = out["fp32_model"]["bead_steps_per_s"] / out["fp64_exact"]["bead_steps_per_s"]
+    # force error on one state (the plain run's last state)
+    pos, vel = systems["fp64_exact"].get_state()
+    ref = systems["fp64_exact"]
+    ref.set_state(pos, vel)
+    f64 = ref.forces()[0]
+    sur = systems["fp32_model"]
+    sur.set_state(pos, vel)
+    fs = sur.forces()[0]
+    ref.far_drop(True)
+    fn = ref.forces()[0]
+    ref.far_drop(False)
+    rel = lambda a: float(np.sqrt(np.sum((a - f64) ** 2) / np.sum(f64 ** 2)))
+    out["force_rel_rms_err"] = rel(fs)
+    out["far_field_share_of_force"] = rel(fn)  # error of leaving the far field out
+    out["workload"] = (f"trimer_MVM x{ns}, C=800uM c={args.sur_salt}mM, uncut Yukawa (-e 0 over the box), "
+                       f"far field beyond 20 sigma centre distance, NH MD, skin {args.skin}")
+    out["beads"] = systems["fp64_exact"].n_beads
+    return out
+
+
+def bench_surrogate(args):
+    res = run_surrogate(args, args.steps)
+    return {
+        "metric": "bead-steps/s", "value": res["fp32_model"]["bead_steps_per_s"], "unit": "bead-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 100, "ms_per_step": res["fp32_model"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pairs, bf16 far-field network (f32 accumulate), f64 integrate", "data": "synthetic",
+        "config": {"workload": res["workload"] + "; far field from the network", "subunits": args.sur_subunits,
+                   "beads": res["beads"], "parallelism": "single GPU"},
+        "plain_fp64": res["fp64_exact"], "surrogate_fp32": res["fp32_model"],
+        "speedup_vs_fp64_exact": res["speedup_vs_fp64_exact"], "force_rel_rms_err": res["force_rel_rms_err"],
+        "far_field_share_of_force": res["far_field_share_of_force"], "training": res["train"],
+        "training_wall_s": res["train_wall_s"],
+    }
+
+
 def bench_md(args):
     md = run_gpu(args)
     st = md["stats"]
@@ -308,6 +395,19 @@ def bench_md(args):
                         "fp32_pair_energy_per_bead": [f32["e_start"], f32["e_end"]],
                         "fp64_energy_drift_per_bead": md["e_end"] - md["e_start"],
                         "fp32_pair_energy_drift_per_bead": f32["e_end"] - f32["e_start"]}
+        for r in (bd, f32):
+            r["system"].close()
+        sur = run_surrogate(args, args.sur_steps)
+        out["extra"]["surrogate_5000"] = {
+            "workload": sur["workload"],
+            "fp64_exact_bead_steps_per_s": sur["fp64_exact"]["bead_steps_per_s"],
+            "fp32_model_bead_steps_per_s": sur["fp32_model"]["bead_steps_per_s"],
+            "speedup_vs_fp64_exact": sur["speedup_vs_fp64_exact"],
+            "force_rel_rms_err": sur["force_rel_rms_err"],
+            "far_field_share_of_force": sur["far_field_share_of_force"],
+            "k_far_ms": {"exact": sur["fp64_exact"]["kernels_ms"]["k_far"], "model": sur["fp32_model"]["kernels_ms"]["k_far"]},
+            "audits": sur["fp32_model"]["audits"], "training_wall_s": sur["train_wall_s"],
+            "val_rel_err_force": sur["train"]["val_rel_err_force"]}
     for k in ("system",):
         md.pop(k, None)
     if not args.no_cpu:
