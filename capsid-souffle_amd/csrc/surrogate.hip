@@ -270,7 +270,17 @@ __global__ void k_gather_rows(const float* src, const int* idx, float* dst, int 
 // ---- BF16 MFMA inference -----------------------------------------------------------------
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// SiLU for inference: z / (1 + e^-z) with the hardware exp and reciprocal.
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+// Two floats to packed bf16 (round to nearest even, v_cvt_pk_bf16_f32).
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
 
 // FP32 -> bf16 bits, round to nearest even (finite inputs).
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
@@ -300,73 +310,81 @@ __global__ void k_pack_weights(const float* p, unsigned short* w, float* bias) {
 
 constexpr int kMlpWaves = 4;
 constexpr int kMlpThreads = 64 * kMlpWaves;
-constexpr int kTileLd = kFLdH;          // activation tile row stride (bf16)
-constexpr int kTileElems = 32 * kTileLd;  // one wave's 32-row tile
+constexpr int kRowsPerWave = 64;           // pairs (rows) a wave carries through the network
+constexpr int kTileLd = kFLdH;             // activation tile row stride (bf16)
+constexpr int kTileElems = kRowsPerWave * kTileLd;
 constexpr size_t kMlpLds = (size_t)kFPackN * 2 + (size_t)kMlpWaves * kTileElems * 2 + (3 * kFHid + kFOutPad) * 4;
 
-// One wave runs the network on the 32 rows of its LDS tile h (features in columns 0..31,
-// bf16). The 16x16x32 MFMA operand layout: lane l holds A[l%16][8(l/16)..+7] and
-// B[8(l/16)..+7][l%16]; the result D[4(l/16)+i][l%16], i = 0..3. Returns the output block:
-// out[rb][i] = network output (row rb*16 + 4(l/16) + i, column l%16; columns < 6 used).
-// Every wave of the workgroup must call it together (it synchronises the workgroup).
-__device__ __forceinline__ void mlp_wave(const unsigned short* W, const float* bias, unsigned short* h, int lane,
-                                         f32x4 out[2]) {
+// One wave runs the network on the 64 rows of its LDS tile h (row = pair, features in columns
+// 0..31, bf16). Each layer is computed transposed, D = W H^T, with the weights as the A operand
+// and the activations as the B operand of v_mfma_f32_16x16x32_bf16: lane l holds
+// A[l%16][8(l/16)..+7] (a weight row), B[8(l/16)..+7][l%16] (a pair's inputs) and
+// D[4(l/16)+i][l%16] (four consecutive outputs of one pair), so every operand read is 16 bytes
+// and every activation write-back 8 bytes. Returns out[pt][i] = output 4(l/16)+i of pair
+// 16 pt + l%16 (outputs 0..5 used). Every wave of the workgroup must call it together.
+__device__ __forceinline__ void mlp_tile64(const unsigned short* W, const float* bias, unsigned short* h, int lane,
+                                           f32x4 out[4]) {
   const int r16 = lane & 15, kq = lane >> 4;
-  f32x4 acc[2][8];
-  // layer 1: K = 32
+  f32x4 acc[8][4];
+  // every layer's accumulators start at its bias (outputs 16 mt + 4 kq .. +3)
 #pragma unroll
-  for (int rb = 0; rb < 2; rb++)
+  for (int mt = 0; mt < 8; mt++) {
+    const float4 bv = *reinterpret_cast<const float4*>(bias + mt * 16 + 4 * kq);
 #pragma unroll
-    for (int nt = 0; nt < 8; nt++) acc[rb][nt] = f32x4{0, 0, 0, 0};
-  {
-    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(h + r16 * kTileLd + 8 * kq);
-    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(h + (16 + r16) * kTileLd + 8 * kq);
+    for (int pt = 0; pt < 4; pt++) acc[mt][pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
+  }
+  {  // layer 1: K = 32
+    bf16x8 b[4];
 #pragma unroll
-    for (int nt = 0; nt < 8; nt++) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(W + kFPackW1 + (nt * 16 + r16) * kFLd1 + 8 * kq);
-      acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, acc[0][nt], 0, 0, 0);
-      acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc[1][nt], 0, 0, 0);
+    for (int pt = 0; pt < 4; pt++) b[pt] = *reinterpret_cast<const bf16x8*>(h + (pt * 16 + r16) * kTileLd + 8 * kq);
+#pragma unroll
+    for (int mt = 0; mt < 8; mt++) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + kFPackW1 + (mt * 16 + r16) * kFLd1 + 8 * kq);
+#pragma unroll
+      for (int pt = 0; pt < 4; pt++) acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[pt], acc[mt][pt], 0, 0, 0);
     }
   }
   __syncthreads();  // every lane has read its layer input
 #pragma unroll
   for (int layer = 0; layer < 3; layer++) {
-    // bias + SiLU, back to the tile as bf16 (row 4 kq + i, column nt*16 + r16)
+    // SiLU, back to the tile as bf16: pair 16 pt + r16, outputs 16 mt + 4 kq .. +3
 #pragma unroll
-    for (int rb = 0; rb < 2; rb++)
+    for (int mt = 0; mt < 8; mt++)
 #pragma unroll
-      for (int nt = 0; nt < 8; nt++) {
-        const float bv = bias[layer * kFHid + nt * 16 + r16];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-          h[(rb * 16 + 4 * kq + i) * kTileLd + nt * 16 + r16] = bf16_bits(silu(acc[rb][nt][i] + bv));
+      for (int pt = 0; pt < 4; pt++) {
+        const unsigned lo = pack_bf16(silu_fast(acc[mt][pt][0]), silu_fast(acc[mt][pt][1]));
+        const unsigned hi = pack_bf16(silu_fast(acc[mt][pt][2]), silu_fast(acc[mt][pt][3]));
+        *reinterpret_cast<uint2*>(h + (pt * 16 + r16) * kTileLd + mt * 16 + 4 * kq) = make_uint2(lo, hi);
       }
     __syncthreads();
     const int wofs = layer == 0 ? kFPackW2 : layer == 1 ? kFPackW3 : kFPackW4;
-    const int nts = layer == 2 ? 1 : 8;
+    const int mts = layer == 2 ? 1 : 8;
+    const float* bnext = bias + (layer + 1) * kFHid;
 #pragma unroll
-    for (int rb = 0; rb < 2; rb++)
+    for (int mt = 0; mt < 8; mt++) {
+      if (mt >= mts) break;
+      const float4 bv = *reinterpret_cast<const float4*>(bnext + mt * 16 + 4 * kq);
 #pragma unroll
-      for (int nt = 0; nt < 8; nt++) acc[rb][nt] = f32x4{0, 0, 0, 0};
+      for (int pt = 0; pt < 4; pt++) acc[mt][pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
+    }
 #pragma unroll
     for (int ks = 0; ks < 4; ks++) {
-      const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(h + r16 * kTileLd + 32 * ks + 8 * kq);
-      const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(h + (16 + r16) * kTileLd + 32 * ks + 8 * kq);
+      bf16x8 b[4];
 #pragma unroll
-      for (int nt = 0; nt < 8; nt++) {
-        if (nt >= nts) break;
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(W + wofs + (nt * 16 + r16) * kFLdH + 32 * ks + 8 * kq);
-        acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b, acc[0][nt], 0, 0, 0);
-        acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc[1][nt], 0, 0, 0);
+      for (int pt = 0; pt < 4; pt++)
+        b[pt] = *reinterpret_cast<const bf16x8*>(h + (pt * 16 + r16) * kTileLd + 32 * ks + 8 * kq);
+#pragma unroll
+      for (int mt = 0; mt < 8; mt++) {
+        if (mt >= mts) break;
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + wofs + (mt * 16 + r16) * kFLdH + 32 * ks + 8 * kq);
+#pragma unroll
+        for (int pt = 0; pt < 4; pt++) acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[pt], acc[mt][pt], 0, 0, 0);
       }
     }
     __syncthreads();
   }
-  const float b4 = bias[3 * kFHid + r16];
 #pragma unroll
-  for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) out[rb][i] = acc[rb][0][i] + b4;
+  for (int pt = 0; pt < 4; pt++) out[pt] = acc[0][pt];
 }
 
 // Loads the packed weights and biases into LDS.
@@ -375,6 +393,20 @@ __device__ __forceinline__ void load_net(const unsigned short* wsrc, const float
   uint4* d = reinterpret_cast<uint4*>(W);
   for (int i = threadIdx.x; i < kFPackN / 8; i += blockDim.x) d[i] = s[i];
   for (int i = threadIdx.x; i < 3 * kFHid + kFOutPad; i += blockDim.x) bias[i] = bsrc[i];
+}
+
+// Writes one tile row: 13 features as bf16, zero to column 31.
+__device__ __forceinline__ void put_features(unsigned short* row, const float* x, bool valid) {
+  unsigned w[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const float a = (valid && 2 * k < kFIn) ? x[2 * k] : 0.0f;
+    const float b = (valid && 2 * k + 1 < kFIn) ? x[2 * k + 1] : 0.0f;
+    w[k] = pack_bf16(a, b);
+  }
+  uint4* dst = reinterpret_cast<uint4*>(row);
+#pragma unroll
+  for (int k = 0; k < 4; k++) dst[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
 }
 
 // Standalone inference on n rows of 13 features: y [n][6] raw network outputs.
@@ -387,26 +419,24 @@ __global__ __launch_bounds__(kMlpThreads) void k_mlp_rows(const unsigned short* 
   load_net(wpack, bpack, W, bias);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   unsigned short* h = tiles + wave * kTileElems;
-  const int row0 = (blockIdx.x * kMlpWaves + wave) * 32;
-  if (lane < 32) {
+  const int row0 = (blockIdx.x * kMlpWaves + wave) * kRowsPerWave;
+  {
     const int r = row0 + lane;
-    for (int k = 0; k < 32; k++) {
-      const float v = (r < n && k < kFIn) ? x[(size_t)r * kFIn + k] : 0.0f;
-      h[lane * kTileLd + k] = bf16_bits(v);
-    }
+    float xr[kFIn];
+    for (int k = 0; k < kFIn; k++) xr[k] = r < n ? x[(size_t)r * kFIn + k] : 0.0f;
+    put_features(h + lane * kTileLd, xr, r < n);
   }
   __syncthreads();
-  f32x4 out[2];
-  mlp_wave(W, bias, h, lane, out);
+  f32x4 out[4];
+  mlp_tile64(W, bias, h, lane, out);
   const int r16 = lane & 15, kq = lane >> 4;
-  if (r16 < kFOut) {
 #pragma unroll
-    for (int rb = 0; rb < 2; rb++)
+  for (int pt = 0; pt < 4; pt++) {
+    const int r = row0 + pt * 16 + r16;
+    if (r >= n) continue;
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int r = row0 + rb * 16 + 4 * kq + i;
-        if (r < n) y[(size_t)r * kFOut + r16] = out[rb][i];
-      }
+    for (int i = 0; i < 4; i++)
+      if (4 * kq + i < kFOut) y[(size_t)r * kFOut + 4 * kq + i] = out[pt][i];
   }
 }
 
@@ -427,19 +457,19 @@ __global__ void k_frames(Dev d, FarDev f, int fp1, int fp2) {
 }
 
 // Far field of every subunit A by the network: all subunits B of A's replica whose centre
-// bead lies beyond rf, 128 pairs per round (32 per wave). Each pair's force is spread over A's
-// charged beads with weights exp(kappa qbody_a . n) / sum (n: unit vector to B in A's body
-// frame; nearer charges feel the screened field more), and the pair torques are summed; after
-// all pairs a rigid correction w x (qbody_a - qcen), w = M^-1 (tau - tau_spread), restores the
-// network's total torque about the centre bead. Sums run in a fixed order; the per-bead forces
-// go to the lab frame with A's body frame. Persistent: workgroups stride over the subunits.
+// bead lies beyond rf, 256 pairs per round (64 per wave, one per lane). Features, the pair
+// scale and the spread weights are computed in FP32 (the centre-bead separation in FP64).
+// Each pair's force is spread over A's charged beads with weights exp(kappa qbody_a . n) / sum
+// (n: unit vector to B in A's body frame; nearer charges feel the screened field more) and the
+// pair torques are summed; after all pairs a rigid correction w x (qbody_a - qcen),
+// w = M^-1 (tau - tau_spread), restores the network's total torque about the centre bead. Sums
+// run in a fixed order; the per-bead forces go to the lab frame with A's body frame.
+// Persistent: workgroups stride over the subunits.
 __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
   extern __shared__ unsigned char lds[];
   unsigned short* W = reinterpret_cast<unsigned short*>(lds);
   unsigned short* tiles = W + kFPackN;
   float* bias = reinterpret_cast<float*>(tiles + kMlpWaves * kTileElems);
-  __shared__ float scl[kMlpWaves][32];
-  __shared__ float wts[kMlpWaves][32][kFMaxQ];
   __shared__ double red[kMlpWaves][3 * kFMaxQ + 3];
   __shared__ FarSpec spec;
   if (threadIdx.x == 0) spec = *f.spec;
@@ -449,87 +479,139 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
   const int r16 = lane & 15, kq = lane >> 4;
   unsigned short* h = tiles + wave * kTileElems;
   const int nq = d.topo.nq;
-  const float ymu = r16 < kFOut ? spec.ymu[r16] : 0.0f, ysd = r16 < kFOut ? spec.ysd[r16] : 0.0f;
+  float qb[kFMaxQ][3];
+#pragma unroll
+  for (int a = 0; a < kFMaxQ; a++)
+    for (int c = 0; c < 3; c++) qb[a][c] = a < nq ? (float)spec.qbody[a][c] : 0.0f;
+  // this lane's outputs 4 kq + i: standardisation
+  float ymu[4], ysd[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    ymu[i] = 4 * kq + i < kFOut ? spec.ymu[4 * kq + i] : 0.0f;
+    ysd[i] = 4 * kq + i < kFOut ? spec.ysd[4 * kq + i] : 0.0f;
+  }
+  const float rf = (float)spec.rf, lbf = (float)spec.lb;
   for (int A = blockIdx.x; A < d.Ns; A += gridDim.x) {
     const RepParams& P = d.rp[d.sub_rep[A]];
-    const double box = P.box, hbox = P.hbox, kappa = P.kappa;
+    const double box = P.box, hbox = P.hbox;
+    const float kappa = (float)P.kappa;
     const double4 ca = d.centres[A];
-    double RA[9];
+    float RA[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) RA[k] = (double)f.frames[(size_t)A * 12 + k];
-    // lane r16 < 3: spread force component r16 on each charged bead; 3 <= r16 < 6: torque
-    double accq[kFMaxQ];
+    for (int k = 0; k < 9; k++) RA[k] = f.frames[(size_t)A * 12 + k];
+    // kq == 0 lanes: spread force (3 components x charged beads) and torque x; kq == 1: torque y, z
+    double accq[3][kFMaxQ], acct[3] = {0, 0, 0};
 #pragma unroll
-    for (int a = 0; a < kFMaxQ; a++) accq[a] = 0.0;
-    double acct = 0.0;
-    for (int t0 = 0; t0 < P.ns; t0 += 32 * kMlpWaves) {
-      if (lane < 32) {
-        const int t = t0 + wave * 32 + lane;
-        float x[kFIn];
-        float sc = 0.0f;
-        if (t < P.ns && P.s0 + t != A) {
-          const int B = P.s0 + t;
-          const double4 cb = d.centres[B];
-          const double cx = min_img_f(ca.x - cb.x, box, hbox), cy = min_img_f(ca.y - cb.y, box, hbox),
-                       cz = min_img_f(ca.z - cb.z, box, hbox);
-          const double d2 = cx * cx + cy * cy + cz * cz;
-          if (d2 > P.far_rf2) {
-            // d = R_A^T (c_B - c_A), Q = R_A^T R_B
-            const double dv[3] = {-(RA[0] * cx + RA[3] * cy + RA[6] * cz), -(RA[1] * cx + RA[4] * cy + RA[7] * cz),
-                                  -(RA[2] * cx + RA[5] * cy + RA[8] * cz)};
-            double RB[9], Q[9];
-            for (int k = 0; k < 9; k++) RB[k] = (double)f.frames[(size_t)B * 12 + k];
-            for (int i = 0; i < 3; i++)
-              for (int j = 0; j < 3; j++) Q[3 * i + j] = RA[i] * RB[j] + RA[3 + i] * RB[3 + j] + RA[6 + i] * RB[6 + j];
-            pose_features(dv, Q, spec.rf, kappa, x);
-            sc = (float)far_scale(spec, dv, Q, kappa);
-            // spread weights of A's charged beads for this pair
-            const double dn = sqrt(d2);
-            double ws[kFMaxQ], wsum = 0.0;
-            for (int a = 0; a < nq; a++) {
-              const double proj = (spec.qbody[a][0] * dv[0] + spec.qbody[a][1] * dv[1] + spec.qbody[a][2] * dv[2]) / dn;
-              ws[a] = exp(kappa * proj);
-              wsum += ws[a];
-            }
-            for (int a = 0; a < nq; a++) wts[wave][lane][a] = (float)(ws[a] / wsum);
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int a = 0; a < kFMaxQ; a++) accq[c][a] = 0.0;
+    for (int t0 = 0; t0 < P.ns; t0 += kRowsPerWave * kMlpWaves) {
+      // this lane's pair: features, scale and spread weights
+      const int t = t0 + wave * kRowsPerWave + lane;
+      float x[kFIn], w[kFMaxQ];
+      float sc = 0.0f;
+      if (t < P.ns && P.s0 + t != A) {
+        const int B = P.s0 + t;
+        const double4 cb = d.centres[B];
+        const double cx = min_img_f(ca.x - cb.x, box, hbox), cy = min_img_f(ca.y - cb.y, box, hbox),
+                     cz = min_img_f(ca.z - cb.z, box, hbox);
+        const double d2 = cx * cx + cy * cy + cz * cz;
+        if (d2 > P.far_rf2) {
+          const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
+          const float dv[3] = {-(RA[0] * fx + RA[3] * fy + RA[6] * fz), -(RA[1] * fx + RA[4] * fy + RA[7] * fz),
+                               -(RA[2] * fx + RA[5] * fy + RA[8] * fz)};
+          float RB[9], Q[9];
+#pragma unroll
+          for (int k = 0; k < 9; k++) RB[k] = f.frames[(size_t)B * 12 + k];
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) Q[3 * i + j] = RA[i] * RB[j] + RA[3 + i] * RB[3 + j] + RA[6 + i] * RB[6 + j];
+          const float dd = sqrtf((float)d2), inv = 1.0f / dd;
+          const float n[3] = {dv[0] * inv, dv[1] * inv, dv[2] * inv};
+          const float wr = rf * inv;
+          x[0] = n[0]; x[1] = n[1]; x[2] = n[2];
+          x[3] = wr; x[4] = wr * wr; x[5] = wr * wr * wr;
+          x[6] = Q[0]; x[7] = Q[3]; x[8] = Q[6]; x[9] = Q[1]; x[10] = Q[4]; x[11] = Q[7];
+          x[12] = kappa;
+          // pair scale (far_scale) and spread weights
+          float pa[kFMaxQ], ma = -1e30f, mb = -1e30f;
+#pragma unroll
+          for (int a = 0; a < kFMaxQ; a++) {
+            if (a >= nq) break;
+            pa[a] = qb[a][0] * n[0] + qb[a][1] * n[1] + qb[a][2] * n[2];
+            const float pb = -((Q[0] * qb[a][0] + Q[1] * qb[a][1] + Q[2] * qb[a][2]) * n[0] +
+                               (Q[3] * qb[a][0] + Q[4] * qb[a][1] + Q[5] * qb[a][2]) * n[1] +
+                               (Q[6] * qb[a][0] + Q[7] * qb[a][1] + Q[8] * qb[a][2]) * n[2]);
+            w[a] = pb;  // reuse as scratch for p'_b
+            ma = fmaxf(ma, pa[a]);
+            mb = fmaxf(mb, pb);
           }
+          float sa = 0.0f, sb = 0.0f;
+#pragma unroll
+          for (int a = 0; a < kFMaxQ; a++) {
+            if (a >= nq) break;
+            sb += __expf(kappa * (w[a] - mb));
+            w[a] = __expf(kappa * (pa[a] - ma));
+            sa += w[a];
+          }
+          const float isa = 1.0f / sa;
+#pragma unroll
+          for (int a = 0; a < kFMaxQ; a++) w[a] = a < nq ? w[a] * isa : 0.0f;
+          sc = lbf * (kappa + inv) * inv * __expf(-kappa * (dd - ma - mb)) * sa * sb;
         }
-        for (int k = 0; k < 32; k++) h[lane * kTileLd + k] = bf16_bits(sc != 0.0f && k < kFIn ? x[k] : 0.0f);
-        scl[wave][lane] = sc;
       }
+      if (!(sc > 0.0f)) {
+        sc = 0.0f;
+#pragma unroll
+        for (int a = 0; a < kFMaxQ; a++) w[a] = 0.0f;
+      }
+      put_features(h + lane * kTileLd, x, sc > 0.0f);
       __syncthreads();
-      f32x4 out[2];
-      mlp_wave(W, bias, h, lane, out);
-      if (r16 < kFOut) {
+      f32x4 out[4];
+      mlp_tile64(W, bias, h, lane, out);
 #pragma unroll
-        for (int rb = 0; rb < 2; rb++)
+      for (int pt = 0; pt < 4; pt++) {
+        const int src = pt * 16 + r16;  // the lane that built this pair
+        const float scp = __shfl(sc, src, 64);
+        float wp[kFMaxQ];
 #pragma unroll
-          for (int i = 0; i < 4; i++) {
-            const int row = rb * 16 + 4 * kq + i;
-            const float sc = scl[wave][row];
-            if (sc == 0.0f) continue;
-            const double v = (double)((ymu + ysd * out[rb][i]) * sc);
-            if (r16 < 3) {
-              for (int a = 0; a < nq; a++) accq[a] += v * (double)wts[wave][row][a];
-            } else {
-              acct += v;
-            }
+        for (int a = 0; a < kFMaxQ; a++) wp[a] = __shfl(w[a], src, 64);
+        if (scp == 0.0f) continue;
+        const float v0 = (ymu[0] + ysd[0] * out[pt][0]) * scp, v1 = (ymu[1] + ysd[1] * out[pt][1]) * scp,
+                    v2 = (ymu[2] + ysd[2] * out[pt][2]) * scp, v3 = (ymu[3] + ysd[3] * out[pt][3]) * scp;
+        if (kq == 0) {
+#pragma unroll
+          for (int a = 0; a < kFMaxQ; a++) {
+            accq[0][a] += (double)(v0 * wp[a]);
+            accq[1][a] += (double)(v1 * wp[a]);
+            accq[2][a] += (double)(v2 * wp[a]);
           }
+          acct[0] += (double)v3;
+        } else if (kq == 1) {
+          acct[1] += (double)v0;
+          acct[2] += (double)v1;
+        }
       }
       __syncthreads();
     }
-    // fixed-order sums: the four lanes of a column, then the waves
+    // fixed-order sums over the 16 lanes of a quarter-wave, then the waves
 #pragma unroll
-    for (int a = 0; a < kFMaxQ; a++) {
-      accq[a] += __shfl_down(accq[a], 16, 64);
-      accq[a] += __shfl_down(accq[a], 32, 64);
+    for (int o = 8; o > 0; o >>= 1) {
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int a = 0; a < kFMaxQ; a++) accq[c][a] += __shfl_xor(accq[c][a], o, 64);
+#pragma unroll
+      for (int c = 0; c < 3; c++) acct[c] += __shfl_xor(acct[c], o, 64);
     }
-    acct += __shfl_down(acct, 16, 64);
-    acct += __shfl_down(acct, 32, 64);
-    if (lane < 3) {
-      for (int a = 0; a < nq; a++) red[wave][3 * a + lane] = accq[a];
-    } else if (lane < kFOut) {
-      red[wave][3 * kFMaxQ + lane - 3] = acct;
+    if (lane == 0) {
+      for (int a = 0; a < nq; a++)
+        for (int c = 0; c < 3; c++) red[wave][3 * a + c] = accq[c][a];
+      red[wave][3 * kFMaxQ] = acct[0];
+    } else if (lane == 16) {
+      red[wave][3 * kFMaxQ + 1] = acct[1];
+      red[wave][3 * kFMaxQ + 2] = acct[2];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -537,12 +619,12 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
       for (int a = 0; a < nq; a++)
         for (int c = 0; c < 3; c++) {
           double s = 0.0;
-          for (int w = 0; w < kMlpWaves; w++) s += red[w][3 * a + c];
+          for (int w2 = 0; w2 < kMlpWaves; w2++) s += red[w2][3 * a + c];
           fb[a][c] = s;
         }
       for (int c = 0; c < 3; c++) {
         double s = 0.0;
-        for (int w = 0; w < kMlpWaves; w++) s += red[w][3 * kFMaxQ + c];
+        for (int w2 = 0; w2 < kMlpWaves; w2++) s += red[w2][3 * kFMaxQ + c];
         T[c] = s * spec.rq;
       }
       for (int a = 0; a < nq; a++) {
@@ -552,14 +634,14 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
         Ts[2] += q[0] * fb[a][1] - q[1] * fb[a][0];
       }
       const double dt[3] = {T[0] - Ts[0], T[1] - Ts[1], T[2] - Ts[2]};
-      double w[3];
-      for (int i = 0; i < 3; i++) w[i] = spec.minv[3 * i] * dt[0] + spec.minv[3 * i + 1] * dt[1] + spec.minv[3 * i + 2] * dt[2];
+      double wv[3];
+      for (int i = 0; i < 3; i++) wv[i] = spec.minv[3 * i] * dt[0] + spec.minv[3 * i + 1] * dt[1] + spec.minv[3 * i + 2] * dt[2];
       for (int a = 0; a < nq; a++) {
         const double r[3] = {spec.qbody[a][0] - spec.qcen[0], spec.qbody[a][1] - spec.qcen[1], spec.qbody[a][2] - spec.qcen[2]};
-        const double g[3] = {fb[a][0] + (w[1] * r[2] - w[2] * r[1]), fb[a][1] + (w[2] * r[0] - w[0] * r[2]),
-                             fb[a][2] + (w[0] * r[1] - w[1] * r[0])};
+        const double g[3] = {fb[a][0] + (wv[1] * r[2] - wv[2] * r[1]), fb[a][1] + (wv[2] * r[0] - wv[0] * r[2]),
+                             fb[a][2] + (wv[0] * r[1] - wv[1] * r[0])};
         double* o = d.qfar + 3 * ((size_t)A * nq + a);
-        for (int i = 0; i < 3; i++) o[i] = RA[3 * i] * g[0] + RA[3 * i + 1] * g[1] + RA[3 * i + 2] * g[2];
+        for (int i = 0; i < 3; i++) o[i] = (double)RA[3 * i] * g[0] + (double)RA[3 * i + 1] * g[1] + (double)RA[3 * i + 2] * g[2];
       }
     }
     __syncthreads();
@@ -646,6 +728,7 @@ int launch_pack_weights(const float* params, unsigned short* wpack, float* bias,
 }
 
 int launch_mlp_rows(const unsigned short* wpack, const float* bias, int n, const float* x, float* y, hipStream_t st) {
+  const int rows_per_block = kRowsPerWave * kMlpWaves;
   static bool lds_set = false;
   if (!lds_set) {
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mlp_rows), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -655,7 +738,6 @@ int launch_mlp_rows(const unsigned short* wpack, const float* bias, int n, const
     }
     lds_set = true;
   }
-  const int rows_per_block = 32 * kMlpWaves;
   hipLaunchKernelGGL(k_mlp_rows, dim3((n + rows_per_block - 1) / rows_per_block), dim3(kMlpThreads), kMlpLds, st, wpack,
                      bias, n, x, y);
   return check_launch_far();
