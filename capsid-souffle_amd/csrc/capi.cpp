@@ -168,6 +168,67 @@ struct sf_handle {
     return SF_OK;
   }
 
+  // Cell grids of the list build (cut Yukawa): per replica g cells per axis with edge at least
+  // the list radius (+2e-3 for the FP32 tests), g = 1 when fewer than 3 fit; global cell
+  // arrays sized for all replicas.
+  int setup_cells() {
+    if (!d.cells) return SF_OK;
+    long long nlj = 0, nq = 0;
+    for (int r = 0; r < R; r++) {
+      RepParams& P = rp[r];
+      // at most about two beads per cell on average: finer grids only cost memory
+      auto grid = [&](double radius, long long n) {
+        int g = (int)std::floor(P.box / (radius + 2e-3));
+        const int gmax = (int)std::floor(std::cbrt(2.0 * (double)std::max(n, 1ll)));
+        g = std::min(g, gmax);
+        return g < 3 ? 1 : g;
+      };
+      P.g_lj = grid(P.rl_lj, (long long)P.nb);
+      P.g_q = grid(prm[r].ecut_el + prm[r].skin, (long long)P.nqr);
+      P.inv_a_lj = (float)(P.g_lj / P.box);
+      P.inv_a_q = (float)(P.g_q / P.box);
+      P.c0_lj = (int)nlj;
+      P.c0_q = (int)nq;
+      nlj += (long long)P.g_lj * P.g_lj * P.g_lj;
+      nq += (long long)P.g_q * P.g_q * P.g_q;
+    }
+    if (nlj > (1ll << 30) || nq > (1ll << 30)) { set_error("too many list-build cells"); return SF_E_ARG; }
+    if (nlj != d.ncell_lj) {
+      release(d.cnt_lj);
+      if (alloc(&d.cnt_lj, (size_t)nlj)) return SF_E_HIP;
+      d.ncell_lj = (int)nlj;
+      const int want = std::max(16, (int)(4.0 * Nb / nlj) + 4);
+      if (int rc = grow_cells(std::max(want, d.ccap_lj), 0, true)) return rc;
+      HIPCHK(hipMemsetAsync(d.cnt_lj, 0, (size_t)nlj * sizeof(int), st));
+    }
+    if (nq != d.ncell_q) {
+      release(d.cnt_q);
+      if (alloc(&d.cnt_q, (size_t)nq)) return SF_E_HIP;
+      d.ncell_q = (int)nq;
+      const int want = std::max(16, (int)(4.0 * std::max(Nq, 1) / nq) + 4);
+      if (int rc = grow_cells(0, std::max(want, d.ccap_q), true)) return rc;
+      HIPCHK(hipMemsetAsync(d.cnt_q, 0, (size_t)nq * sizeof(int), st));
+    }
+    return SF_OK;
+  }
+
+  // (Re)allocates the cell records for capacities of at least lj / q slots per cell.
+  int grow_cells(int lj, int q, bool exact = false) {
+    if (lj > d.ccap_lj || (exact && lj > 0)) {
+      const int cap = exact ? lj : std::max(lj + lj / 4 + 4, 2 * d.ccap_lj);
+      release(d.cell_lj);
+      if (alloc(&d.cell_lj, (size_t)d.ncell_lj * cap)) return SF_E_HIP;
+      d.ccap_lj = cap;
+    }
+    if (q > d.ccap_q || (exact && q > 0)) {
+      const int cap = exact ? q : std::max(q + q / 4 + 4, 2 * d.ccap_q);
+      release(d.cell_q);
+      if (alloc(&d.cell_q, (size_t)std::max(d.ncell_q, 1) * cap)) return SF_E_HIP;
+      d.ccap_q = cap;
+    }
+    return SF_OK;
+  }
+
   int upload_rp() {
     HIPCHK(hipMemcpyAsync((void*)d.rp, rp.data(), R * sizeof(RepParams), hipMemcpyHostToDevice, st));
     return SF_OK;
@@ -203,14 +264,18 @@ struct sf_handle {
   int check_and_fix() {
     std::vector<RepState> h;
     if (int rc = read_states(h)) return rc;
-    int need_lj = 0, need_q = 0, need_c = 0, deformed = 0;
+    int need_lj = 0, need_q = 0, need_c = 0, deformed = 0, need_clj = 0, need_cq = 0;
     for (auto& s : h) {
       need_lj = std::max(need_lj, s.need_lj);
       need_q = std::max(need_q, s.need_q);
       need_c = std::max(need_c, s.need_cand);
+      need_clj = std::max(need_clj, s.need_cell_lj);
+      need_cq = std::max(need_cq, s.need_cell_q);
       deformed |= s.deformed;
     }
-    if (!need_lj && !need_q && !need_c && !deformed) return 0;
+    if (!need_lj && !need_q && !need_c && !deformed && !need_clj && !need_cq) return 0;
+    if (need_clj || need_cq)
+      if (int rc = grow_cells(need_clj, need_cq)) return rc;
     if (need_c) {
       set_error("more than 1024 candidate subunits near one subunit at a list build");
       return SF_E_OVERFLOW;
@@ -228,7 +293,7 @@ struct sf_handle {
   int clear_flags_force_rebuild() {
     std::vector<RepState> h;
     if (int rc = read_states(h)) return rc;
-    for (auto& s : h) { s.need_lj = s.need_q = s.need_cand = s.deformed = 0; s.rebuild = 1; }
+    for (auto& s : h) { s.need_lj = s.need_q = s.need_cand = s.deformed = s.need_cell_lj = s.need_cell_q = 0; s.rebuild = 1; }
     HIPCHK(hipMemcpyAsync(d.rs, h.data(), R * sizeof(RepState), hipMemcpyHostToDevice, st));
     return SF_OK;
   }
@@ -266,7 +331,7 @@ struct sf_handle {
       if (rc == 0) return SF_OK;
       std::vector<RepState> h;
       if ((rc = read_states(h))) return rc;
-      for (auto& s : h) { s.need_lj = s.need_q = s.need_cand = s.deformed = 0; }
+      for (auto& s : h) { s.need_lj = s.need_q = s.need_cand = s.deformed = s.need_cell_lj = s.need_cell_q = 0; }
       HIPCHK(hipMemcpyAsync(d.rs, h.data(), R * sizeof(RepState), hipMemcpyHostToDevice, st));
     }
     set_error("list capacity did not converge");
@@ -304,6 +369,8 @@ struct sf_handle {
   int g_cap[2] = {0, 0};
   int g_tile = 0;
   bool g_far = false, g_drop = false;
+  const void* g_cell[4] = {nullptr, nullptr, nullptr, nullptr};
+  int g_ccap[2] = {0, 0};
   std::vector<hipEvent_t> events;
 
   void drop_graph() {
@@ -329,7 +396,8 @@ struct sf_handle {
     // buffer may come back at the same address, so the capacities are part of the key
     if (!gexec || g_n != n || g_tm != timing || g_key[0] != d.lj_list || g_key[1] != d.q_list ||
         g_cap[0] != d.lj_cap || g_cap[1] != d.q_cap || g_tile != d.tile || g_far != far_model ||
-        g_drop != far_drop) {
+        g_drop != far_drop || g_cell[0] != d.cell_lj || g_cell[1] != d.cnt_lj || g_cell[2] != d.cell_q ||
+        g_cell[3] != d.cnt_q || g_ccap[0] != d.ccap_lj || g_ccap[1] != d.ccap_q) {
       drop_graph();
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -342,6 +410,8 @@ struct sf_handle {
       if (ei != hipSuccess) { gexec = nullptr; set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return SF_E_HIP; }
       g_n = n; g_tm = timing; g_key[0] = d.lj_list; g_key[1] = d.q_list;
       g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap; g_tile = d.tile; g_far = far_model; g_drop = far_drop;
+      g_cell[0] = d.cell_lj; g_cell[1] = d.cnt_lj; g_cell[2] = d.cell_q; g_cell[3] = d.cnt_q;
+      g_ccap[0] = d.ccap_lj; g_ccap[1] = d.ccap_q;
     }
     HIPCHK(hipGraphLaunch(gexec, st));
     return SF_OK;
@@ -530,6 +600,8 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
   d.gx = gbuf; d.gy = gbuf + nb; d.gz = gbuf + 2 * nb;
   d.lj_cap = 0; d.q_cap = 0;
   if (h->grow_lists(24, nq > 0 ? 48 : 1)) return fail(nullptr);
+  d.cells = (!split && (long long)h->Nb < (1ll << 23)) ? 1 : 0;
+  if (h->setup_cells() || h->upload_rp()) return fail(nullptr);
   if (hipMemset(d.qfar, 0, 3 * (size_t)std::max(h->Nq, 1) * sizeof(double)) != hipSuccess ||
       hipMemset(d.efar, 0, (size_t)std::max(h->Nq, 1) * sizeof(double)) != hipSuccess ||
       hipMemset(fbuf, 0, 3 * nb * sizeof(double)) != hipSuccess || hipMemset(gbuf, 0, 3 * nb * sizeof(double)) != hipSuccess ||
@@ -602,11 +674,10 @@ int sf_forces(sf_handle* h, double* f, double* fs, double* fb, double* fe, doubl
   if (!h) { set_error("null handle"); return SF_E_ARG; }
   const size_t n3 = 3 * (size_t)h->Nb;
   if (!h->out_buf && h->alloc(&h->out_buf, 5 * n3)) return SF_E_HIP;
-  Dev d = h->d;
-  d.out_f = h->out_buf; d.out_s = h->out_buf + n3; d.out_b = h->out_buf + 2 * n3;
-  d.out_e = h->out_buf + 3 * n3; d.out_lj = h->out_buf + 4 * n3;
   for (int attempt = 0; attempt < 16; attempt++) {
-    d.lj_list = h->d.lj_list; d.q_list = h->d.q_list; d.lj_cap = h->d.lj_cap; d.q_cap = h->d.q_cap;
+    Dev d = h->d;  // capacities and buffers as regrown by the previous attempt
+    d.out_f = h->out_buf; d.out_s = h->out_buf + n3; d.out_b = h->out_buf + 2 * n3;
+    d.out_e = h->out_buf + 3 * n3; d.out_lj = h->out_buf + 4 * n3;
     if (int rc = launch_centres(d, h->st)) return rc;
     if (int rc = launch_rebuild(d, 1, h->st)) return rc;
     if (int rc = h->launch_far(0)) return rc;
@@ -616,7 +687,7 @@ int sf_forces(sf_handle* h, double* f, double* fs, double* fb, double* fe, doubl
     if (rc == 0) break;
     std::vector<RepState> s;
     if ((rc = h->read_states(s))) return rc;
-    for (auto& x : s) { x.need_lj = x.need_q = x.need_cand = x.deformed = 0; }
+    for (auto& x : s) { x.need_lj = x.need_q = x.need_cand = x.deformed = x.need_cell_lj = x.need_cell_q = 0; }
     HIPCHK(hipMemcpyAsync(h->d.rs, s.data(), h->R * sizeof(RepState), hipMemcpyHostToDevice, h->st));
   }
   double* outs[5] = {f, fs, fb, fe, flj};
@@ -764,6 +835,7 @@ int sf_set_skin(sf_handle* h, int replica, double skin) {
     }
   for (int r = r0; r < r1; r++) h->put_skin(r, skin);
   h->fill_bounds();
+  if (int rc = h->setup_cells()) return rc;
   if (int rc = h->upload_rp()) return rc;
   return h->request_rebuild();
 }
@@ -817,6 +889,7 @@ int sf_autotune(sf_handle* h, const sf_tune_opts* opt, sf_tune_result* res, doub
     for (int r = 0; r < h->R; r++) h->put_skin(r, std::min(skin, h->max_skin(r)));
     h->fill_bounds();
     h->d.tile = tile;
+    if (int rc = h->setup_cells()) return rc;
     if (int rc = h->upload_rp()) return rc;
     return h->request_rebuild();
   };

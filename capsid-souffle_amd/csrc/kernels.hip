@@ -147,6 +147,11 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
   if (l == d.topo.centre) d.centres[s] = p;
   const int qs = d.topo.qslot[l];
   if (qs >= 0) d.qpos[(size_t)s * d.topo.nq + qs] = p;
+  // cell occupancies of this step's (possible) list build start from zero
+  if (d.cells) {
+    for (int c = b; c < d.ncell_lj; c += d.Nb) d.cnt_lj[c] = 0;
+    for (int c = b; c < d.ncell_q; c += d.Nb) d.cnt_q[c] = 0;
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -533,6 +538,506 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
   const double wl = wave_sum(active ? (double)cnt_lj : 0.0), wq = wave_sum(active ? (double)cnt_q : 0.0);
   __syncthreads();
   double* red = reinterpret_cast<double*>(stage);
+  if ((tid & 63) == 0) { red[2 * (tid >> 6)] = wl; red[2 * (tid >> 6) + 1] = wq; }
+  __syncthreads();
+  if (tid == 0) {
+    d.part[(size_t)s * kPartStride + 7] = red[0] + red[2];
+    d.part[(size_t)s * kPartStride + 8] = red[1] + red[3];
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Cell-list build (cut Yukawa). Each replica's box is cut into g^3 cells per grid (g = 1 or
+// g >= 3; cell edge >= the list radius): one grid for all beads (LJ/WCA lists) and one for the
+// charged beads (Yukawa lists). A fill kernel drops every bead of a rebuilding replica into its
+// cell as an FP32 record; a query kernel, one workgroup per subunit, scans the 27 cells around
+// each bead with the FP32 list radius plus 1e-3 sigma (lists are supersets of the exact ones;
+// entries beyond a cutoff contribute nothing) and keeps each list sorted by bead id, the order
+// in which the reference's update_pairlist appends (src/functions.cpp:93-110), so that forces
+// do not depend on when lists were built.
+
+__device__ __forceinline__ float wrapf(float x, float box, float hbox) {
+  return x - box * floorf((x + hbox) / box);
+}
+
+__device__ __forceinline__ int cell_of(float x, float hbox, float inv_a, int g) {
+  const int c = (int)((x + hbox) * inv_a);
+  return c < 0 ? 0 : (c >= g ? g - 1 : c);
+}
+
+__global__ void k_cells_clear(Dev d) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < d.ncell_lj; c += gridDim.x * blockDim.x) d.cnt_lj[c] = 0;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < d.ncell_q; c += gridDim.x * blockDim.x) d.cnt_q[c] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_cell_fill(Dev d, int force_rebuild) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= d.Nb) return;
+  const int s = b / d.topo.np, l = b - s * d.topo.np;
+  const int r = d.sub_rep[s];
+  RepState& S = d.rs[r];
+  if (!force_rebuild && S.rebuild == 0) return;
+  const RepParams& P = d.rp[r];
+  const double4 p = d.pos[b];
+  const float box = (float)P.box, hbox = (float)P.hbox;
+  const float x = wrapf((float)p.x, box, hbox), y = wrapf((float)p.y, box, hbox), z = wrapf((float)p.z, box, hbox);
+  const float4 rec = make_float4(x, y, z, __int_as_float(b));
+  {
+    const int g = P.g_lj;
+    const int c = P.c0_lj + (cell_of(z, hbox, P.inv_a_lj, g) * g + cell_of(y, hbox, P.inv_a_lj, g)) * g +
+                  cell_of(x, hbox, P.inv_a_lj, g);
+    const int slot = atomicAdd(&d.cnt_lj[c], 1);
+    if (slot < d.ccap_lj) d.cell_lj[(size_t)c * d.ccap_lj + slot] = rec;
+    else atomicMax(&S.need_cell_lj, slot + 1);
+  }
+  if (d.topo.qslot[l] >= 0) {
+    const int g = P.g_q;
+    const int c = P.c0_q + (cell_of(z, hbox, P.inv_a_q, g) * g + cell_of(y, hbox, P.inv_a_q, g)) * g +
+                  cell_of(x, hbox, P.inv_a_q, g);
+    const int slot = atomicAdd(&d.cnt_q[c], 1);
+    if (slot < d.ccap_q) d.cell_q[(size_t)c * d.ccap_q + slot] = rec;
+    else atomicMax(&S.need_cell_q, slot + 1);
+  }
+}
+
+constexpr int kCqRecords = 1024;  // LDS records (other subunits) of the cells around one subunit
+constexpr int kCqCells = 512;     // cells of the box around one subunit's stencils
+constexpr int kCqLjSlots = 32;    // LDS slots per bead for its sorted LJ list
+constexpr int kCqQMax = 8;        // charged beads per template with LDS Yukawa slots
+constexpr int kCqQSlots = 256;    // LDS slots per charged bead for its Yukawa list
+
+// In-place insertion sort of one list column (stride n) in global memory; used only when a
+// list outgrew its LDS slots.
+__device__ void sort_column(int* a, size_t stride, int n) {
+  for (int k = 1; k < n; k++) {
+    const int v = a[(size_t)k * stride];
+    int p = k;
+    while (p > 0 && a[(size_t)(p - 1) * stride] > v) {
+      a[(size_t)p * stride] = a[(size_t)(p - 1) * stride];
+      p--;
+    }
+    a[(size_t)p * stride] = v;
+  }
+}
+
+__device__ __forceinline__ int wrap_cell(int c, int g) { return c < 0 ? c + g : (c >= g ? c - g : c); }
+
+// Signed cell offset of c from cc on a periodic axis of g cells (nearest image).
+__device__ __forceinline__ int rel_cell(int c, int cc, int g) {
+  int r = c - cc;
+  if (2 * r > g) r -= g;
+  else if (2 * r < -g) r += g;
+  return r;
+}
+
+// LJ / WCA list radius^2 test of bead i against candidate record e (FP32, +1e-3 sigma).
+struct LjTest {
+  float ix, iy, iz, si, ratt2, skinf, box, hbox;
+  unsigned long long mask_i;
+  __device__ __forceinline__ bool operator()(const float4& e, int tj, float sj) const {
+    const float dx = min_imgf(ix - e.x, box, hbox), dy = min_imgf(iy - e.y, box, hbox), dz = min_imgf(iz - e.z, box, hbox);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if ((mask_i >> tj) & 1ull) return r2 < ratt2;
+    const float w = (float)kWcaFactor * 0.5f * (si + sj) + skinf;
+    return r2 < w * w;
+  }
+};
+
+// Local box of cells around a subunit on one axis: cells cc + lo .. cc + lo + span - 1
+// (periodic), or the whole axis when the stencils wrap around it (full).
+struct CellAxis {
+  int cc, lo, span, g;
+  bool full;
+  __device__ __forceinline__ int local(int rel) const { return full ? wrap_cell(cc + rel, g) : rel - lo; }
+  __device__ __forceinline__ int global(int l) const { return full ? l : wrap_cell(cc + lo + l, g); }
+};
+
+// The cells of a g^3 grid (first cell c0) that the 27-cell stencils of a workgroup's
+// participating beads touch are marked, counted, prefix-summed and their records copied into
+// LDS in local-cell order, leaving out records with ids in [excl_lo, excl_hi) (the workgroup's
+// own subunit); cnum then holds the kept records of each cell, which start at coff. Returns
+// false (uniformly) when the box or the records do not fit.
+__device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt, int cap, int c0, int g, float inv_a, float hbox,
+                             float cx, float cy, float cz, bool part, float px, float py, float pz, int tid,
+                             CellAxis* ax, int* rel_out, int* cnum, int* cfull, int* coff, float4* rec, int* red_i,
+                             int excl_lo, int excl_hi) {
+  const int ccx = cell_of(cx, hbox, inv_a, g), ccy = cell_of(cy, hbox, inv_a, g), ccz = cell_of(cz, hbox, inv_a, g);
+  int rx = 0, ry = 0, rz = 0;
+  if (part) {
+    rx = rel_cell(cell_of(px, hbox, inv_a, g), ccx, g);
+    ry = rel_cell(cell_of(py, hbox, inv_a, g), ccy, g);
+    rz = rel_cell(cell_of(pz, hbox, inv_a, g), ccz, g);
+  }
+  rel_out[0] = rx; rel_out[1] = ry; rel_out[2] = rz;
+  // block min / max of the relative cells (non-participants contribute the centre cell)
+  int mn[3] = {rx, ry, rz}, mx[3] = {rx, ry, rz};
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      mn[k] = min(mn[k], __shfl_xor(mn[k], o, 64));
+      mx[k] = max(mx[k], __shfl_xor(mx[k], o, 64));
+    }
+  if ((tid & 63) == 0)
+    for (int k = 0; k < 3; k++) { red_i[6 * (tid >> 6) + k] = mn[k]; red_i[6 * (tid >> 6) + 3 + k] = mx[k]; }
+  __syncthreads();
+  if (tid < 3) {
+    const int lo = min(red_i[tid], red_i[6 + tid]) - 1, hi = max(red_i[3 + tid], red_i[9 + tid]) + 1;
+    CellAxis& A = ax[tid];
+    A.g = g;
+    A.cc = tid == 0 ? ccx : (tid == 1 ? ccy : ccz);
+    if (g == 1) { A.full = true; A.lo = 0; A.span = 1; }
+    else if (hi - lo + 1 >= g) { A.full = true; A.lo = 0; A.span = g; }
+    else { A.full = false; A.lo = lo; A.span = hi - lo + 1; }
+  }
+  __syncthreads();
+  const int sx = ax[0].span, sy = ax[1].span, sz = ax[2].span;
+  const int total = sx * sy * sz;
+  if (total > kCqCells) return false;  // uniform: every thread reads the same spans
+  for (int lc = tid; lc < total; lc += kBlock) cnum[lc] = 0;
+  __syncthreads();
+  if (part) {  // mark the stencil cells (cnum used as a flag first)
+    for (int dz = -1; dz <= 1; dz++) {
+      if (g == 1 && dz != 0) continue;
+      const int lz = ax[2].local(rz + dz);
+      for (int dy = -1; dy <= 1; dy++) {
+        if (g == 1 && dy != 0) continue;
+        const int ly = ax[1].local(ry + dy);
+        for (int dx = -1; dx <= 1; dx++) {
+          if (g == 1 && dx != 0) continue;
+          cnum[(lz * sy + ly) * sx + ax[0].local(rx + dx)] = 1;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // counts of the marked cells, then an exclusive prefix sum (each thread owns 4 cells)
+  int loc[4], sum = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int lc = 4 * tid + k;
+    int n = 0;
+    if (lc < total && cnum[lc]) {
+      const int gl = c0 + (ax[2].global(lc / (sx * sy)) * g + ax[1].global((lc / sx) % sy)) * g + ax[0].global(lc % sx);
+      n = min(cnt[gl], cap);
+    }
+    loc[k] = n;
+    sum += n;
+  }
+  int incl = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o, 64);
+    if ((tid & 63) >= o) incl += v;
+  }
+  __syncthreads();
+  if ((tid & 63) == 63) red_i[tid >> 6] = incl;
+  __syncthreads();
+  const int base = incl - sum + ((tid >> 6) ? red_i[0] : 0);
+  const int all = red_i[0] + red_i[1];
+  int acc = base;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int lc = 4 * tid + k;
+    if (lc < kCqCells) { coff[lc] = acc; cfull[lc] = loc[k]; cnum[lc] = 0; }
+    acc += loc[k];
+  }
+  if (tid == 0) coff[kCqCells] = all;
+  if (all > kCqRecords) return false;  // uniform
+  __syncthreads();
+  // copy the records of other subunits: each thread takes records r = tid, tid + 128, ...,
+  // finding a record's cell by binary search of the offsets; kept records are packed at the
+  // start of their cell's range
+  for (int rr = tid; rr < all; rr += kBlock) {
+    int lo = 0, hi = total - 1;  // last local cell with coff <= rr (and a record there)
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (coff[mid] <= rr) lo = mid; else hi = mid - 1;
+    }
+    while (cfull[lo] == 0 || coff[lo] + cfull[lo] <= rr) lo--;  // skip empty cells sharing the offset
+    const int gl = c0 + (ax[2].global(lo / (sx * sy)) * g + ax[1].global((lo / sx) % sy)) * g + ax[0].global(lo % sx);
+    const float4 v = cells[(size_t)gl * cap + (rr - coff[lo])];
+    const int j = __float_as_int(v.w);
+    if (j >= excl_lo && j < excl_hi) continue;
+    rec[coff[lo] + atomicAdd(&cnum[lo], 1)] = v;
+  }
+  __syncthreads();
+  return true;
+}
+
+// The 27 cells around (cx, cy, cz) of a g^3 grid starting at c0 (periodic; g >= 3), or the
+// single cell (g = 1).
+template <typename F>
+__device__ __forceinline__ void for_stencil(int c0, int g, int cx, int cy, int cz, F&& f) {
+  if (g == 1) { f(c0); return; }
+  for (int dz = -1; dz <= 1; dz++) {
+    const int zz = wrap_cell(cz + dz, g);
+    for (int dy = -1; dy <= 1; dy++) {
+      const int yy = wrap_cell(cy + dy, g);
+      for (int dx = -1; dx <= 1; dx++) f(c0 + (zz * g + yy) * g + wrap_cell(cx + dx, g));
+    }
+  }
+}
+
+// Lists of bead gi (active thread) straight from the global cells, written unsorted and then
+// sorted in place: the path of a workgroup whose neighbourhood does not fit in LDS.
+__device__ __forceinline__ void cell_query_direct(const Dev& d, const RepParams& P, int s, int gi, int tid, const LjTest T,
+                                  const int* ttype, const float* tsig, int* counts) {
+  int cnt_lj = 0, cnt_q = 0;
+  const int np = d.topo.np, nq = d.topo.nq;
+  const float inv_np = 1.0f / (float)np;
+  cnt_lj = 0;
+  {
+    const int g = P.g_lj;
+    for_stencil(P.c0_lj, g, cell_of(T.ix, T.hbox, P.inv_a_lj, g), cell_of(T.iy, T.hbox, P.inv_a_lj, g),
+                cell_of(T.iz, T.hbox, P.inv_a_lj, g), [&](int c) {
+      const int n = min(d.cnt_lj[c], d.ccap_lj);
+      const float4* cell = d.cell_lj + (size_t)c * d.ccap_lj;
+      for (int m = 0; m < n; m++) {
+        const float4 e = cell[m];
+        const int j = __float_as_int(e.w);
+        const int lj = local_of(j, np, inv_np);
+        if (j - lj == s * np || !T(e, ttype[lj], tsig[lj])) continue;
+        if (cnt_lj < d.lj_cap) d.lj_list[(size_t)cnt_lj * d.Nb + gi] = j;
+        cnt_lj++;
+      }
+    });
+    sort_column(d.lj_list + gi, d.Nb, min(cnt_lj, d.lj_cap));
+  }
+  cnt_q = 0;
+  const int qi = d.topo.qslot[tid];
+  if (qi >= 0) {
+    const int gq = s * nq + qi;
+    const float rq = sqrtf((float)P.rc_q_list2) + 1e-3f;
+    const float rq2 = rq * rq;
+    const int g = P.g_q;
+    for_stencil(P.c0_q, g, cell_of(T.ix, T.hbox, P.inv_a_q, g), cell_of(T.iy, T.hbox, P.inv_a_q, g),
+                cell_of(T.iz, T.hbox, P.inv_a_q, g), [&](int c) {
+      const int n = min(d.cnt_q[c], d.ccap_q);
+      const float4* cell = d.cell_q + (size_t)c * d.ccap_q;
+      for (int m = 0; m < n; m++) {
+        const float4 e = cell[m];
+        const int j = __float_as_int(e.w);
+        const float dx = min_imgf(T.ix - e.x, T.box, T.hbox), dy = min_imgf(T.iy - e.y, T.box, T.hbox),
+                    dz = min_imgf(T.iz - e.z, T.box, T.hbox);
+        if (j == gi || !(dx * dx + dy * dy + dz * dz < rq2)) continue;
+        if (cnt_q < d.q_cap) d.q_list[(size_t)cnt_q * d.Nq + gq] = j;
+        cnt_q++;
+      }
+    });
+    sort_column(d.q_list + gq, d.Nq, min(cnt_q, d.q_cap));
+  }
+  counts[0] = cnt_lj;
+  counts[1] = cnt_q;
+}
+
+// One workgroup per subunit of a rebuilding replica: the cells its beads' stencils touch are
+// gathered into LDS (all beads for the LJ lists, then the charged beads for the Yukawa lists);
+// each bead keeps its LJ hits insertion-sorted in LDS, the Yukawa hits are ranked by all
+// threads together. Neighbourhoods that do not fit in LDS take the direct path.
+__global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild) {
+  __shared__ float4 rec[kCqRecords];
+  __shared__ int cnum[kCqCells];
+  __shared__ int cfull[kCqCells];
+  __shared__ int coff[kCqCells + 1];
+  __shared__ CellAxis ax[3];
+  __shared__ int red_i[16];
+  __shared__ int ljbuf[kCqLjSlots * kBlock];
+  __shared__ int qbuf[kCqQMax * kCqQSlots];
+  __shared__ int qn[kCqQMax];
+  __shared__ float4 qsh[kCqQMax];
+  __shared__ int qcell[kCqQMax][3];
+  __shared__ float tsig[kBlock];
+  __shared__ int ttype[kBlock];
+  __shared__ unsigned long long tmask[kMaxTypes];
+  __shared__ double red[4];
+  const int s = blockIdx.x, tid = threadIdx.x;
+  const int r = d.sub_rep[s];
+  RepState& S = d.rs[r];
+  if (!force_rebuild && S.rebuild == 0) return;
+  const RepParams& P = d.rp[r];
+  const int np = d.topo.np, nq = d.topo.nq;
+  const bool active = tid < np;
+  const int gi = s * np + tid;
+  if (active) { tsig[tid] = (float)d.topo.sigma[tid]; ttype[tid] = d.topo.type[tid]; }
+  if (tid < kMaxTypes) tmask[tid] = d.topo.att_mask[tid];
+  if (tid < kCqQMax) qn[tid] = 0;
+  const float box = (float)P.box, hbox = (float)P.hbox;
+  const float inv_np = 1.0f / (float)np;
+  double4 pi = make_double4(0, 0, 0, 0);
+  if (active) pi = d.pos[gi];
+  const double4 cs = d.centres[s];
+  const float cx = wrapf((float)cs.x, box, hbox), cy = wrapf((float)cs.y, box, hbox), cz = wrapf((float)cs.z, box, hbox);
+  LjTest T;
+  T.ix = wrapf((float)pi.x, box, hbox); T.iy = wrapf((float)pi.y, box, hbox); T.iz = wrapf((float)pi.z, box, hbox);
+  T.box = box; T.hbox = hbox;
+  T.si = active ? (float)d.topo.sigma[tid] : 0.0f;
+  T.mask_i = active ? d.topo.att_mask[d.topo.type[tid]] : 0ull;
+  const float ratt = (float)(P.ecut + P.skin) + 1e-3f;
+  T.ratt2 = ratt * ratt;
+  T.skinf = (float)P.skin + 1e-3f;
+  __syncthreads();
+  int cnt_lj = 0, cnt_q = 0;
+  int rel[3];
+  // ---- LJ / WCA: the neighbourhood's records of other subunits, gathered into LDS
+  const bool lj_ok = gather_cells(d.cell_lj, d.cnt_lj, d.ccap_lj, P.c0_lj, P.g_lj, P.inv_a_lj, hbox, cx, cy, cz, active,
+                                  T.ix, T.iy, T.iz, tid, ax, rel, cnum, cfull, coff, rec, red_i, s * np, (s + 1) * np);
+  bool direct = !lj_ok;
+  if (lj_ok && active) {
+    // the local box in registers
+    const int g = ax[0].g, sx = ax[0].span, sy = ax[1].span;
+    const bool fx = ax[0].full, fy = ax[1].full, fz = ax[2].full;
+    const int ox = fx ? ax[0].cc : -ax[0].lo, oy = fy ? ax[1].cc : -ax[1].lo, oz = fz ? ax[2].cc : -ax[2].lo;
+    const int d0 = g == 1 ? 0 : -1, d1 = g == 1 ? 0 : 1;
+    bool over = false;
+    for (int pass = 0; pass < 2; pass++) {  // pass 1 only when the LDS slots overflowed
+      if (pass == 1) {
+        if (!over) break;
+        cnt_lj = 0;
+      }
+      for (int dz = d0; dz <= d1; dz++) {
+        const int lz = fz ? wrap_cell(oz + rel[2] + dz, g) : rel[2] + dz + oz;
+        for (int dy = d0; dy <= d1; dy++) {
+          const int ly = fy ? wrap_cell(oy + rel[1] + dy, g) : rel[1] + dy + oy;
+          for (int dx = d0; dx <= d1; dx++) {
+            const int lxx = fx ? wrap_cell(ox + rel[0] + dx, g) : rel[0] + dx + ox;
+            const int lc = (lz * sy + ly) * sx + lxx;
+            const int m0 = coff[lc], m1 = m0 + cnum[lc];
+            for (int m = m0; m < m1; m++) {
+              const float4 e = rec[m];
+              const int j = __float_as_int(e.w);
+              const int lj = local_of(j, np, inv_np);
+              if (!T(e, ttype[lj], tsig[lj])) continue;
+              if (pass == 1) {
+                if (cnt_lj < d.lj_cap) d.lj_list[(size_t)cnt_lj * d.Nb + gi] = j;
+              } else if (cnt_lj < kCqLjSlots) {
+                int p = cnt_lj;
+                while (p > 0 && ljbuf[(p - 1) * kBlock + tid] > j) {
+                  ljbuf[p * kBlock + tid] = ljbuf[(p - 1) * kBlock + tid];
+                  p--;
+                }
+                ljbuf[p * kBlock + tid] = j;
+              } else {
+                over = true;
+              }
+              cnt_lj++;
+            }
+          }
+        }
+      }
+    }
+    if (!over) {
+      for (int k = 0; k < cnt_lj && k < d.lj_cap; k++) d.lj_list[(size_t)k * d.Nb + gi] = ljbuf[k * kBlock + tid];
+    } else {  // rare: the list went to global memory in pass 1; sort it there
+      sort_column(d.lj_list + gi, d.Nb, min(cnt_lj, d.lj_cap));
+    }
+  }
+  // ---- Yukawa: charged beads j != i within ecut_el + skin. Work items (charged bead, stencil
+  // cell) spread over all threads; hits are appended in LDS and ranked afterwards, so the
+  // list order (ascending id) does not depend on the append order.
+  if (nq > 0 && !direct) {
+    __syncthreads();
+    const int qi = active ? d.topo.qslot[tid] : -1;
+    const int g = P.g_q;
+    if (qi >= 0 && qi < kCqQMax) {
+      qsh[qi] = make_float4(T.ix, T.iy, T.iz, __int_as_float(gi));
+      qcell[qi][0] = cell_of(T.ix, hbox, P.inv_a_q, g);
+      qcell[qi][1] = cell_of(T.iy, hbox, P.inv_a_q, g);
+      qcell[qi][2] = cell_of(T.iz, hbox, P.inv_a_q, g);
+    }
+    __syncthreads();
+    const float rq = sqrtf((float)P.rc_q_list2) + 1e-3f;
+    const float rq2 = rq * rq;
+    const int nql = min(nq, kCqQMax);
+    const int ncell = g == 1 ? 1 : 27;
+    for (int it = tid; it < nql * ncell; it += kBlock) {
+      const int a = it / ncell, k = it - a * ncell;
+      int c = P.c0_q;
+      if (g > 1)
+        c += (wrap_cell(qcell[a][2] + k / 9 - 1, g) * g + wrap_cell(qcell[a][1] + (k / 3) % 3 - 1, g)) * g +
+             wrap_cell(qcell[a][0] + k % 3 - 1, g);
+      const float4 qa = qsh[a];
+      const int self = __float_as_int(qa.w);
+      const int n = min(d.cnt_q[c], d.ccap_q);
+      const float4* cell = d.cell_q + (size_t)c * d.ccap_q;
+      for (int m0 = 0; m0 < n; m0 += 8) {  // eight records requested together
+        float4 e[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+          if (m0 + u < n) e[u] = cell[m0 + u];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          if (m0 + u >= n) break;
+          const int j = __float_as_int(e[u].w);
+          const float dx = min_imgf(qa.x - e[u].x, box, hbox), dy = min_imgf(qa.y - e[u].y, box, hbox),
+                      dz = min_imgf(qa.z - e[u].z, box, hbox);
+          if (j == self || !(dx * dx + dy * dy + dz * dz < rq2)) continue;
+          const int slot = atomicAdd(&qn[a], 1);
+          if (slot < kCqQSlots) qbuf[a * kCqQSlots + slot] = j;
+        }
+      }
+    }
+    __syncthreads();
+    // rank sort of each list: an entry's rank is the number of smaller ids in it
+    for (int a = 0; a < nql; a++) {
+      const int n = qn[a];
+      if (n > kCqQSlots) continue;  // overflowed: its bead takes the direct path below
+      const int gq = s * nq + a;
+      const int* buf = qbuf + a * kCqQSlots;
+      for (int k = tid; k < n; k += kBlock) {
+        const int v = buf[k];
+        int rank = 0;
+        for (int m = 0; m < n; m++) rank += buf[m] < v ? 1 : 0;
+        if (rank < d.q_cap) d.q_list[(size_t)rank * d.Nq + gq] = v;
+      }
+    }
+    if (qi >= 0) {
+      const int gq = s * nq + qi;
+      if (qi < kCqQMax && qn[qi] <= kCqQSlots) {
+        cnt_q = qn[qi];
+      } else {  // rare: this bead's list straight from the global cells, sorted in place
+        cnt_q = 0;
+        for_stencil(P.c0_q, g, cell_of(T.ix, hbox, P.inv_a_q, g), cell_of(T.iy, hbox, P.inv_a_q, g),
+                    cell_of(T.iz, hbox, P.inv_a_q, g), [&](int c) {
+          const int n = min(d.cnt_q[c], d.ccap_q);
+          const float4* cell = d.cell_q + (size_t)c * d.ccap_q;
+          for (int m = 0; m < n; m++) {
+            const float4 e = cell[m];
+            const int j = __float_as_int(e.w);
+            const float dx = min_imgf(T.ix - e.x, box, hbox), dy = min_imgf(T.iy - e.y, box, hbox),
+                        dz = min_imgf(T.iz - e.z, box, hbox);
+            if (j == gi || !(dx * dx + dy * dy + dz * dz < rq2)) continue;
+            if (cnt_q < d.q_cap) d.q_list[(size_t)cnt_q * d.Nq + gq] = j;
+            cnt_q++;
+          }
+        });
+        sort_column(d.q_list + gq, d.Nq, min(cnt_q, d.q_cap));
+      }
+      d.q_cnt[gq] = cnt_q;
+      if (cnt_q > d.q_cap) atomicMax(&S.need_q, cnt_q);
+    }
+  }
+  if (direct) {  // uniform: the neighbourhood did not fit in LDS
+    __syncthreads();
+    if (active) {
+      int* cts = ljbuf + 2 * tid;  // two LDS words per thread, free by now
+      cell_query_direct(d, P, s, gi, tid, T, ttype, tsig, cts);
+      cnt_lj = cts[0];
+      cnt_q = cts[1];
+      const int qi = d.topo.qslot[tid];
+      if (qi >= 0) {
+        d.q_cnt[s * nq + qi] = cnt_q;
+        if (cnt_q > d.q_cap) atomicMax(&S.need_q, cnt_q);
+      }
+    }
+  }
+  if (active) {
+    d.lj_cnt[gi] = cnt_lj;
+    d.xb[gi] = pi;
+    if (cnt_lj > d.lj_cap) atomicMax(&S.need_lj, cnt_lj);
+  }
+  // list sizes of this subunit for the statistics
+  const double wl = wave_sum(active ? (double)cnt_lj : 0.0), wq = wave_sum(active ? (double)cnt_q : 0.0);
   if ((tid & 63) == 0) { red[2 * (tid >> 6)] = wl; red[2 * (tid >> 6) + 1] = wq; }
   __syncthreads();
   if (tid == 0) {
@@ -1164,7 +1669,17 @@ int launch_integrate(const Dev& d, int mode, hipStream_t st) {
 }
 
 int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st) {
-  hipLaunchKernelGGL(k_rebuild, dim3(d.Ns), dim3(kBlock), 0, st, d, force_rebuild);
+  if (!d.cells) {
+    hipLaunchKernelGGL(k_rebuild, dim3(d.Ns), dim3(kBlock), 0, st, d, force_rebuild);
+    return check_launch();
+  }
+  if (force_rebuild) {  // outside the step loop no integrate kernel has cleared the cells
+    hipLaunchKernelGGL(k_cells_clear, dim3(256), dim3(256), 0, st, d);
+    if (int rc = check_launch()) return rc;
+  }
+  hipLaunchKernelGGL(k_cell_fill, dim3((d.Nb + 255) / 256), dim3(256), 0, st, d, force_rebuild);
+  if (int rc = check_launch()) return rc;
+  hipLaunchKernelGGL(k_cell_query, dim3(d.Ns), dim3(kBlock), 0, st, d, force_rebuild);
   return check_launch();
 }
 

@@ -123,6 +123,10 @@ struct RepParams {
   int s0, ns, b0, nb, q0, nqr;   // subunit / bead / charged-slot ranges
   int q_chain, brownian, pair_fp32, split;  // split: uncut Yukawa as near (lists) + far field
   double far_rf2;                // far field: centre-bead distance^2 beyond this
+  // cell grids of the list build (cut Yukawa): g cells per axis (1 or >= 3), first cell index
+  // of the replica in the global cell arrays, 1 / cell edge
+  int g_lj, g_q, c0_lj, c0_q;
+  float inv_a_lj, inv_a_q;
   uint32_t seed, mult6n;         // engine seed; 16807^(6 nb) mod (2^31-1)
   const double* qq;              // [nq][nq] q_i q_j lb
   const double* yk;              // [nq][nq] Yukawa size factor exp(k shc)/((1+k s_i/2)(1+k s_j/2))
@@ -144,6 +148,7 @@ struct RepState {
   int need_lj, need_q;  // longest list seen when it exceeded the capacity
   int need_cand;        // candidate subunits when they exceeded kMaxCand
   int deformed;         // a bead left the centre-bead bound at a list build
+  int need_cell_lj, need_cell_q;  // largest cell occupancy seen when it exceeded the capacity
   long long step, rebuilds, lj_entries, q_entries;
 };
 
@@ -167,6 +172,11 @@ struct Dev {
   double4* qpos;       // [Nq] charged-bead positions (subunit-major, charged-slot order)
   double* qfar;        // [Nq][3] far-field Yukawa force on each charged bead (split mode)
   double* efar;        // [Nq] far-field Yukawa energy of each charged bead, pairs to higher ids
+  // cell-list build: bead records (x, y, z, id) per cell, fixed capacity, occupancy counts
+  float4* cell_lj; int* cnt_lj;
+  float4* cell_q; int* cnt_q;
+  int ccap_lj, ccap_q, ncell_lj, ncell_q;
+  int cells;           // 1: lists come from the cell build (cut Yukawa)
   int lj_cap, q_cap;
   int Nb, Ns, Nq, R;
   int split;           // 1: every replica uses the uncut near/far Yukawa split
