@@ -61,6 +61,7 @@ struct sf_handle {
   double4* s_pos = nullptr; double* s_v = nullptr; double* s_f = nullptr; double* s_g = nullptr;
   RepState* s_rs = nullptr;
   double* out_buf = nullptr;  // 5 x Nb x 3 for sf_forces
+  OligoBuf olig{};            // oligomer analysis buffers, allocated at its first call
   double* tab = nullptr;      // per-replica qq / yk / bd_amp tables
 
   template <typename T>
@@ -982,6 +983,28 @@ int sf_get_far_stats(sf_handle* h, sf_far_stats* o) {
   o->audits = h->far_audits; o->last_err = h->far_last; o->max_err = h->far_max;
   o->using_model = h->far_model ? 1 : 0; o->fell_back = h->far_fell_back ? 1 : 0;
   return SF_OK;
+}
+
+int sf_oligomers(sf_handle* h, double contact, int* label, int* mass_spectrum) {
+  if (!h) { set_error("null handle"); return SF_E_ARG; }
+  if (!(contact > 0) || !std::isfinite(contact)) { set_error("contact distance must be positive"); return SF_E_ARG; }
+  OligoBuf& o = h->olig;
+  if (!o.parent) {
+    if (h->alloc(&o.ext, h->R) || h->alloc(&o.parent, h->Ns) || h->alloc(&o.size, h->Ns) ||
+        h->alloc(&o.label, h->Ns) || h->alloc(&o.spectrum, h->Ns))
+      return SF_E_HIP;
+  }
+  if (int rc = launch_centres(h->d, h->st)) return rc;
+  if (int rc = launch_oligomers(h->d, contact, o, h->st)) return rc;
+  std::vector<int> lab(h->Ns), ms(h->Ns);
+  HIPCHK(hipMemcpyAsync(lab.data(), o.label, h->Ns * sizeof(int), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(ms.data(), o.spectrum, h->Ns * sizeof(int), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (label) std::memcpy(label, lab.data(), h->Ns * sizeof(int));
+  if (mass_spectrum) std::memcpy(mass_spectrum, ms.data(), h->Ns * sizeof(int));
+  int n = 0;
+  for (int s = 0; s < h->Ns; s++) n += ms[s];
+  return n;
 }
 
 }  // extern "C"

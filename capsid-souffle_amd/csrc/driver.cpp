@@ -41,6 +41,9 @@ struct Opts {
   int B = 20, M = 1000, W = 1000, w = 1000000, N = 4, m = 1;
   double skin = 0.3;       // GPU Verlet skin (no reference counterpart; -L is the reference's list)
   int fp32 = 0;            // --fp32 1: FP32 pair arithmetic
+  double far_rf = 0;       // --far-rf: with -e 0, centre distance beyond which pairs are far field
+                           // (0: 2.5 template diameters)
+  double oligomers = 0;    // --oligomers <contact>: oligomer statistics at movie steps (0: off)
 };
 
 bool parse_bool(const std::string& s) { return s == "1" || s == "true" || s == "yes" || s == "on"; }
@@ -51,7 +54,8 @@ int parse(int argc, char** argv, Opts& o) {
     std::string a = argv[i];
     if (a == "-h" || a == "--help") {
       std::cout << "Usage:\nsouffle-gpu <options>  (flags of the reference: -D -f -C -c -S -s -b -T -t -r -d -q -Q -K -e\n"
-                   "  -R -X -V -E -B -L -M -W -w -N -p -a -G -m -H; GPU only: --skin <sigma>, --fp32 <0|1>)\n";
+                   "  -R -X -V -E -B -L -M -W -w -N -p -a -G -m -H; GPU only: --skin <sigma>, --fp32 <0|1>,\n"
+                   "  --far-rf <sigma> (with -e 0), --oligomers <contact sigma>)\n";
       return 1;
     }
     std::string key, val;
@@ -107,6 +111,8 @@ int parse(int argc, char** argv, Opts& o) {
     else if (k == "H") o.H = atof(v.c_str());
     else if (k == "skin") o.skin = atof(v.c_str());
     else if (k == "fp32") o.fp32 = atoi(v.c_str());
+    else if (k == "far-rf") o.far_rf = atof(v.c_str());
+    else if (k == "oligomers") o.oligomers = atof(v.c_str());
     else { std::cerr << "unknown option -" << k << "\n"; return -1; }
   }
   return 0;
@@ -177,7 +183,6 @@ int main(int argc, char** argv) {
   if (pr != 0) return pr < 0 ? 1 : 0;
   if (o.p) { std::cerr << "encapsulation big beads (-p) are not part of this engine\n"; return 1; }
   if (o.X) { std::cerr << "cluster start (-X) needs infiles/xyz_T4, which this engine does not read\n"; return 1; }
-  if (o.e == 0) { std::cerr << "uncut Yukawa (-e 0) is not supported by the list engine\n"; return 1; }
   mkdir("outfiles", 0755);
   const bool brownian = o.response == 'b';
   std::ofstream traj("outfiles/energy.out", std::ios_base::app);
@@ -211,6 +216,9 @@ int main(int argc, char** argv) {
   if (sf_params_derive(tmpl, &cli, &prm)) return fail("parameters");
   prm.skin = o.skin;
   prm.pair_fp32 = o.fp32;
+  // -e 0: Yukawa over all charged pairs of the box, near field from lists and far field by
+  // subunit pairs beyond far_rf (sf_create takes 0 as 2.5 template diameters)
+  prm.far_rf = o.far_rf;
 
   // model.parameters.out (src/md.cpp:135-180)
   const double Boltzmann = 1.3806e-23, Avagadro = 6.022e23;
@@ -325,6 +333,17 @@ int main(int argc, char** argv) {
     return 0;
   };
   if (energy_row(0, false)) return 1;
+  // oligomer statistics (src/analysis.cpp:143-186, 267-285): one row per movie frame, and the
+  // frame average of (oligomers of size k) x k written to analysis.ms at the end
+  std::ofstream olig_out;
+  std::vector<int> olig_label(ns), olig_ms(ns);
+  std::vector<double> ms_weighted(ns, 0.0);
+  long long olig_frames = 0;
+  if (o.oligomers > 0) {
+    olig_out.open("outfiles/oligomers.out", std::ios_base::app);
+    olig_out << "time" << std::setw(15) << "oligomers" << std::setw(15) << "largest" << std::setw(15)
+             << "mass_spectrum(1.." << ns << ")" << std::endl;
+  }
 
   const double nsteps_d = totaltime / o.dt;
   long long a = loopStart;
@@ -355,6 +374,20 @@ int main(int argc, char** argv) {
       vcom = st.vcom;
       if (energy_row(a, true)) return 1;
     }
+    if (o.oligomers > 0 && a % o.M == 0) {
+      const int n = sf_oligomers(h, o.oligomers, olig_label.data(), olig_ms.data());
+      if (n < 0) return fail("sf_oligomers");
+      int largest = 0;
+      for (int k = 0; k < ns; k++)
+        if (olig_ms[k]) largest = k + 1;
+      olig_out << a << std::setw(15) << n << std::setw(15) << largest;
+      for (int k = 0; k < ns; k++) {
+        olig_out << std::setw(8) << olig_ms[k];
+        ms_weighted[k] += (double)olig_ms[k] * (k + 1);
+      }
+      olig_out << std::endl;
+      olig_frames++;
+    }
     if (a % o.M == 0) {  // src/md.cpp:647-669
       sf_get_state(h, pos.data(), nullptr);
       ofile << "ITEM: TIMESTEP" << std::endl << a << std::endl << "ITEM: NUMBER OF ATOMS" << std::endl << N << std::endl
@@ -374,6 +407,11 @@ int main(int argc, char** argv) {
     a++;
   }
   std::cout << std::endl;
+  if (olig_frames > 0) {
+    std::ofstream msdata("outfiles/analysis.ms");
+    for (int k = 0; k < ns; k++) msdata << ms_weighted[k] / olig_frames << std::setw(15);
+    msdata << std::endl;
+  }
   sf_destroy(h);
   sf_template_free(tmpl);
   return 0;

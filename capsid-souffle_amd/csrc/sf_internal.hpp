@@ -185,6 +185,16 @@ struct Dev {
 
 enum { MODE_FORCES = 0, MODE_MD = 1, MODE_BD = 2 };
 
+// Device buffers of the oligomer analysis (analysis.hip): per subunit of the handle, and the
+// per-replica bead extent.
+struct OligoBuf {
+  unsigned long long* ext;  // [R] largest squared bead distance from its centre bead (bits)
+  int* parent;              // [Ns] union-find parent (global subunit index)
+  int* size;                // [Ns] oligomer size at its root
+  int* label;               // [Ns] replica-local index of the oligomer's smallest subunit
+  int* spectrum;            // [Ns] per replica (at its first subunit): oligomers by size
+};
+
 // kernel launchers (kernels.hip)
 int launch_centres(const Dev& d, hipStream_t st);
 int launch_integrate(const Dev& d, int mode, hipStream_t st);
@@ -197,6 +207,7 @@ int launch_fill(double* p, size_t n, double v, hipStream_t st);  // a kernel, so
 int launch_apply_com(const Dev& d, hipStream_t st);
 int launch_energy(const Dev& d, hipStream_t st);
 int launch_bd_uniforms_state(uint32_t state, int nb, double* out_dev, const Dev& d, hipStream_t st);
+int launch_oligomers(const Dev& d, double contact, OligoBuf& o, hipStream_t st);  // analysis.hip
 
 // host helpers (topology.cpp)
 uint32_t lcg_pow(uint64_t e);   // 16807^e mod (2^31-1)

@@ -110,6 +110,7 @@ SIGNATURES = {
     "sf_far_audit": (ctypes.c_int, [_P, _D]),
     "sf_far_drop": (ctypes.c_int, [_P, ctypes.c_int]),
     "sf_get_far_stats": (ctypes.c_int, [_P, ctypes.POINTER(SfFarStats)]),
+    "sf_oligomers": (ctypes.c_int, [_P, ctypes.c_double, _I, _I]),
     "sf_last_error": (ctypes.c_char_p, []),
 }
 
@@ -305,6 +306,21 @@ class System:
         s = SfFarStats()
         _err(_lib.sf_get_far_stats(self.ptr, ctypes.byref(s)), "sf_get_far_stats")
         return {k: getattr(s, k) for k, _ in SfFarStats._fields_}
+
+    def oligomers(self, contact=2.0):
+        """Oligomers at the current positions, per replica: a list of (label, mass_spectrum)
+        int32 arrays of that replica's subunit count (label = the smallest subunit of each
+        subunit's oligomer; mass_spectrum[k - 1] = oligomers of k subunits)."""
+        ns = sum(self.n_subunits)
+        lab = np.zeros(ns, dtype=np.int32)
+        ms = np.zeros(ns, dtype=np.int32)
+        _err(_lib.sf_oligomers(self.ptr, float(contact), lab.ctypes.data_as(_I), ms.ctypes.data_as(_I)),
+             "sf_oligomers")
+        out, s0 = [], 0
+        for n in self.n_subunits:
+            out.append((lab[s0:s0 + n].copy(), ms[s0:s0 + n].copy()))
+            s0 += n
+        return out
 
     def close(self):
         if getattr(self, "ptr", None):

@@ -292,6 +292,19 @@ typedef struct {
 
 int sf_get_far_stats(sf_handle* h, sf_far_stats* out);
 
+/* ---- assembly analysis -------------------------------------------------------------------- */
+
+/* Oligomers at the current positions, the mass-spectrum analysis of src/analysis.cpp:143-186
+ * (there run on ovito.lammpstrj frames, contact 2): two subunits of one replica belong to one
+ * oligomer when any of their beads are closer than `contact` under the single-shift minimum
+ * image (src/functions.h:42-56), closed transitively. Outputs, both [total subunits], replica r
+ * in the slots of its subunits (replica-major), either may be NULL:
+ *   label[s]          replica-local index of the smallest subunit in s's oligomer;
+ *   mass_spectrum[.]  at replica r's first slot + k - 1: its oligomers of k subunits (the
+ *                     reference's ms_bin row of one frame).
+ * Returns the total number of oligomers over all replicas, or a negative error code. */
+int sf_oligomers(sf_handle* h, double contact, int* label, int* mass_spectrum);
+
 const char* sf_last_error(void);
 
 #ifdef __cplusplus

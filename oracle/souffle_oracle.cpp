@@ -18,6 +18,7 @@
 //   - NH chain                    src/functions.cpp:33-58, src/thermostat.h:35-52
 //   - pair energies               src/energies.cpp:13-93
 //   - step loop body (NH and BD)  src/md.cpp:400-514, bead updates src/bead.h:72-149
+//   - oligomers (mass spectrum)   src/analysis.cpp:143-186
 // Templated on the real type R: R = long double mirrors the reference's VECTOR3D
 // (src/vector3d.h:22); R = double is the FP64 restatement the GPU path is compared with.
 // Quantities the reference keeps in `double` (edge length, face area, bead energies,
@@ -668,6 +669,46 @@ struct Sys : SysBase {
     last_vcom = (double)av.mag();
   }
   double last_vcom = 0;
+
+  // Oligomers of the current positions, as the mass-spectrum pass of src/analysis.cpp:143-186:
+  // each subunit not yet in an oligomer seeds a new one, which grows round by round over the
+  // members added in the previous round, adding every higher-numbered unassigned subunit that
+  // has a bead within `contact` (dist(...).GetMagnitude() < 2 there) of the member's beads.
+  // label[s] = the seed (smallest subunit) of s's oligomer; ms[k - 1] = oligomers of k subunits.
+  int oligomers(double contact, int* label, int* ms) const {
+    const int np = t.np;
+    std::vector<int> owner(Ns, -1);
+    std::vector<std::vector<int>> olig;
+    for (int i = 0; i < Ns; i++) {
+      if (owner[i] != -1) continue;
+      const int idx = (int)olig.size();
+      olig.push_back(std::vector<int>(1, i));
+      owner[i] = idx;
+      size_t oldsize = 0;
+      while (oldsize < olig[idx].size()) {
+        const size_t n = oldsize;
+        oldsize = olig[idx].size();
+        for (size_t j = n; j < oldsize; j++) {
+          const int g = olig[idx][j];
+          for (int k = i + 1; k < Ns; k++) {
+            if (owner[k] != -1) continue;
+            bool touch = false;
+            for (int m = 0; m < np && !touch; m++)
+              for (int nn = 0; nn < np; nn++)
+                if (dist(g * np + m, k * np + nn).mag() < (R)contact) { touch = true; break; }
+            if (touch) { olig[idx].push_back(k); owner[k] = idx; }
+          }
+        }
+      }
+    }
+    for (int s = 0; s < Ns; s++) {
+      if (label) label[s] = olig[owner[s]][0];
+      if (ms) ms[s] = 0;
+    }
+    if (ms)
+      for (auto& o : olig) ms[o.size() - 1] += 1;
+    return (int)olig.size();
+  }
 };
 
 // single-hinge literal bending force for a 4-bead system (tests): builds a one-subunit
@@ -842,6 +883,13 @@ int orc_hinges(void* hv, int* out, int cap) {
     }
     return n;
   });
+}
+
+// Oligomers at the current positions (contact distance 2 in the reference's analysis):
+// label[Ns] = smallest subunit of each subunit's oligomer, ms[Ns] = oligomers by size
+// (either may be NULL). Returns the number of oligomers.
+int orc_oligomers(void* hv, double contact, int* label, int* ms) {
+  return dispatch((OrcHandle*)hv, [&](auto& s) { return s.oligomers(contact, label, ms); });
 }
 
 // The BD noise stream of the reference: n uniform(-0.5, 0.5) draws of

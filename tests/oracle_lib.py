@@ -36,6 +36,8 @@ def load():
         lib.orc_neighbours.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         lib.orc_hinges.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
         lib.orc_bd_uniforms.argtypes = [ctypes.c_longlong, ctypes.c_int, _D]
+        lib.orc_oligomers.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int)]
         lib.orc_last_error.restype = ctypes.c_char_p
         _lib = lib
     return _lib
@@ -117,6 +119,14 @@ class Oracle:
         buf = (ctypes.c_int * 4096)()
         n = load().orc_hinges(self.ptr, buf, 1024)
         return np.array(buf[:4 * n], dtype=np.int64).reshape(n, 4)
+
+    def oligomers(self, contact=2.0):
+        """(label[Ns], mass_spectrum[Ns], count) of the current positions."""
+        lab = np.zeros(self.Ns, dtype=np.int32)
+        ms = np.zeros(self.Ns, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        n = load().orc_oligomers(self.ptr, contact, lab.ctypes.data_as(ip), ms.ctypes.data_as(ip))
+        return lab, ms, n
 
     def close(self):
         if getattr(self, "ptr", None):

@@ -72,3 +72,35 @@ def make_state(tmpl_xyz, n_sub, box, seed, spacing=None, jitter=0.5, perturb=0.0
     ke = 0.5 * np.sum(m[:, None] * vel * vel)
     vel *= np.sqrt(1.5 * len(pos) * T / ke)
     return pos, vel
+
+
+def _min_image(d, box):
+    return np.where(d > box / 2, d - box, np.where(d < -box / 2, d + box, d))
+
+
+def min_bead_distance(a, b, box):
+    """Smallest single-shift minimum-image distance between the beads of a and of b."""
+    d = _min_image(a[:, None, :] - b[None, :, :], box)
+    return float(np.sqrt(np.einsum("ijk,ijk->ij", d, d).min()))
+
+
+def subunit_line(tmpl_xyz, links, box, seed, touch=1.5, apart=2.5, origin=None):
+    """Subunits in a row along x, randomly rotated: subunit k + 1 slides towards subunit k until
+    their closest beads are `touch` apart when links[k] is true, `apart` otherwise. Returns
+    positions [(len(links) + 1) * n_template_beads, 3], wrapped into the box."""
+    rng = np.random.default_rng(seed)
+    body = tmpl_xyz - tmpl_xyz.mean(axis=0)
+    x0 = np.zeros(3) if origin is None else np.asarray(origin, dtype=float)
+    subs = [body @ random_rotation(rng).T + x0]
+    for link in links:
+        nxt = body @ random_rotation(rng).T
+        target = touch if link else apart
+        xs = subs[-1][:, 0].max() - nxt[:, 0].min() + target + 0.5
+        while True:
+            trial = nxt + subs[-1].mean(axis=0) * [0, 1, 1] + [xs, 0, 0]
+            if min_bead_distance(subs[-1], trial, 1e9) < target:
+                break
+            xs -= 0.005
+        subs.append(trial)
+    pos = np.concatenate(subs)
+    return _min_image(pos, box)

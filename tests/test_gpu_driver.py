@@ -50,3 +50,30 @@ def test_driver_outputs_and_resume(tmp_path):
     rows2 = _rows(out / "energy.out")
     assert [r[0] for r in rows2[len(rows):]] == [0, 100, 150]
     assert (out / "restart_150.out").exists()
+
+
+def test_driver_oligomers_and_uncut_yukawa(tmp_path):
+    os.makedirs(tmp_path / "infiles")
+    shutil.copy(TEMPLATE, tmp_path / "infiles" / "trimer_MVM")
+    _run(tmp_path, ["-T", "200", "-W", "100", "-w", "1000", "-M", "100", "-R", "false", "--oligomers", "3.0"])
+    out = tmp_path / "outfiles"
+    rows = _rows(out / "oligomers.out")
+    assert [r[0] for r in rows] == [0, 100, 200]
+    for r in rows:
+        ms = np.array(r[3:])
+        assert len(ms) == 20 and ms.sum() == r[1] and (np.arange(1, 21) * ms).sum() == 20
+        assert r[2] == np.nonzero(ms)[0].max() + 1
+    # analysis.ms: frame average of (oligomers of size k) x k, as src/analysis.cpp:267-285
+    avg = np.array([float(x) for x in open(out / "analysis.ms").read().split()])
+    want = np.mean([np.array(r[3:]) * np.arange(1, 21) for r in rows], axis=0)
+    assert np.allclose(avg, want, rtol=1e-5) and abs(avg.sum() - 20) < 1e-4
+    # -e 0: the Yukawa over every charged pair of the box (far field beyond 2.5 diameters)
+    d2 = tmp_path / "uncut"
+    os.makedirs(d2 / "infiles")
+    shutil.copy(TEMPLATE, d2 / "infiles" / "trimer_MVM")
+    r = subprocess.run([DRIVER, "-f", "trimer_MVM", "-S", "20", "-C", "200", "-c", "10", "-E", "2.1", "-t", "0.005",
+                        "-e", "0", "-T", "100", "-W", "50", "-R", "false"], cwd=d2, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = _rows(d2 / "outfiles" / "energy.out")
+    assert [x[0] for x in rows] == [0, 0, 50, 100] and all(np.isfinite(x).all() for x in rows)
