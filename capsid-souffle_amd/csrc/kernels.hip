@@ -541,8 +541,8 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
   if ((tid & 63) == 0) { red[2 * (tid >> 6)] = wl; red[2 * (tid >> 6) + 1] = wq; }
   __syncthreads();
   if (tid == 0) {
-    d.part[(size_t)s * kPartStride + 7] = red[0] + red[2];
-    d.part[(size_t)s * kPartStride + 8] = red[1] + red[3];
+    d.part[7 * (size_t)d.Ns + s] = red[0] + red[2];
+    d.part[8 * (size_t)d.Ns + s] = red[1] + red[3];
   }
 }
 
@@ -600,11 +600,14 @@ __global__ __launch_bounds__(256) void k_cell_fill(Dev d, int force_rebuild) {
   }
 }
 
-constexpr int kCqRecords = 1024;  // LDS records (other subunits) of the cells around one subunit
+constexpr int kCqRecords = 512;   // LDS records (other subunits) of the cells around one subunit
 constexpr int kCqCells = 512;     // cells of the box around one subunit's stencils
-constexpr int kCqLjSlots = 32;    // LDS slots per bead for its sorted LJ list
+constexpr int kCqLjSlots = 12;    // LDS slots per bead for its sorted LJ list
 constexpr int kCqQMax = 8;        // charged beads per template with LDS Yukawa slots
-constexpr int kCqQSlots = 256;    // LDS slots per charged bead for its Yukawa list
+constexpr int kCqQSlots = 128;    // LDS slots per charged bead for its Yukawa list (a multiple of 4)
+static_assert(kCqQMax * kCqQSlots * 4 <= kCqRecords * 16, "Yukawa slots live in the record space");
+static_assert(kCqCells <= kCqLjSlots * kBlock, "own-bead cell counts live in the LJ slots");
+static_assert(kCqCells == 4 * kBlock, "the gather gives each thread four cells");
 
 // In-place insertion sort of one list column (stride n) in global memory; used only when a
 // list outgrew its LDS slots.
@@ -652,14 +655,35 @@ struct CellAxis {
   __device__ __forceinline__ int global(int l) const { return full ? l : wrap_cell(cc + lo + l, g); }
 };
 
+// Records 4.. of one cell (sn stored) whose ids are outside [excl_lo, excl_hi), appended to
+// dst from position w (the kept ones among the first four), at most n in all, in stored order.
+__device__ __forceinline__ void copy_cell(const float4* cp, int sn, int n, float4* dst, int w, int excl_lo,
+                                          int excl_hi) {
+  for (int m = 4; m < sn; m += 4) {
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) v[u] = cp[min(m + u, sn - 1)];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = __float_as_int(v[u].w);
+      if (m + u < sn && (j < excl_lo || j >= excl_hi)) {
+        if (w < n) dst[w] = v[u];
+        w++;
+      }
+    }
+  }
+}
+
 // The cells of a g^3 grid (first cell c0) that the 27-cell stencils of a workgroup's
-// participating beads touch are marked, counted, prefix-summed and their records copied into
-// LDS in local-cell order, leaving out records with ids in [excl_lo, excl_hi) (the workgroup's
-// own subunit); cnum then holds the kept records of each cell, which start at coff. Returns
-// false (uniformly) when the box or the records do not fit.
+// participating beads touch are marked, and the records of other subunits in them (the cell's
+// records less the workgroup's own beads, ids in [excl_lo, excl_hi), whose cells the
+// participating beads count themselves) are copied into LDS packed in local-cell order: cell lc
+// holds rec[coff[lc] .. coff[lc] + cnum[lc]), and consecutive cells are contiguous. Each thread
+// owns four consecutive local cells for the counting, the prefix sum and the copy. Returns false
+// (uniformly) when the box or the records do not fit.
 __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt, int cap, int c0, int g, float inv_a, float hbox,
                              float cx, float cy, float cz, bool part, float px, float py, float pz, int tid,
-                             CellAxis* ax, int* rel_out, int* cnum, int* cfull, int* coff, float4* rec, int* red_i,
+                             CellAxis* ax, int* rel_out, int* cnum, int* ownc, int* coff, float4* rec, int* red_i,
                              int excl_lo, int excl_hi) {
   const int ccx = cell_of(cx, hbox, inv_a, g), ccy = cell_of(cy, hbox, inv_a, g), ccz = cell_of(cz, hbox, inv_a, g);
   int rx = 0, ry = 0, rz = 0;
@@ -694,9 +718,9 @@ __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt
   const int sx = ax[0].span, sy = ax[1].span, sz = ax[2].span;
   const int total = sx * sy * sz;
   if (total > kCqCells) return false;  // uniform: every thread reads the same spans
-  for (int lc = tid; lc < total; lc += kBlock) cnum[lc] = 0;
+  for (int lc = tid; lc < total; lc += kBlock) { cnum[lc] = 0; ownc[lc] = 0; }
   __syncthreads();
-  if (part) {  // mark the stencil cells (cnum used as a flag first)
+  if (part) {  // mark the stencil cells (cnum used as a flag first) and count the own bead
     for (int dz = -1; dz <= 1; dz++) {
       if (g == 1 && dz != 0) continue;
       const int lz = ax[2].local(rz + dz);
@@ -709,19 +733,21 @@ __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt
         }
       }
     }
+    atomicAdd(&ownc[(ax[2].local(rz) * sy + ax[1].local(ry)) * sx + ax[0].local(rx)], 1);
   }
   __syncthreads();
-  // counts of the marked cells, then an exclusive prefix sum (each thread owns 4 cells)
-  int loc[4], sum = 0;
+  // records of other subunits per marked cell, then an exclusive prefix sum
+  int loc[4], src[4], gcell[4], sum = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int lc = 4 * tid + k;
-    int n = 0;
+    int n = 0, sn = 0, gl = c0;
     if (lc < total && cnum[lc]) {
-      const int gl = c0 + (ax[2].global(lc / (sx * sy)) * g + ax[1].global((lc / sx) % sy)) * g + ax[0].global(lc % sx);
-      n = min(cnt[gl], cap);
+      gl = c0 + (ax[2].global(lc / (sx * sy)) * g + ax[1].global((lc / sx) % sy)) * g + ax[0].global(lc % sx);
+      sn = min(cnt[gl], cap);
+      n = max(sn - ownc[lc], 0);
     }
-    loc[k] = n;
+    loc[k] = n; src[k] = sn; gcell[k] = gl;
     sum += n;
   }
   int incl = sum;
@@ -735,32 +761,43 @@ __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt
   __syncthreads();
   const int base = incl - sum + ((tid >> 6) ? red_i[0] : 0);
   const int all = red_i[0] + red_i[1];
-  int acc = base;
+  int off[4], acc = base;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int lc = 4 * tid + k;
-    if (lc < kCqCells) { coff[lc] = acc; cfull[lc] = loc[k]; cnum[lc] = 0; }
+    off[k] = acc;
+    if (lc < kCqCells) { coff[lc] = acc; cnum[lc] = loc[k]; }
     acc += loc[k];
   }
   if (tid == 0) coff[kCqCells] = all;
   if (all > kCqRecords) return false;  // uniform
-  __syncthreads();
-  // copy the records of other subunits: each thread takes records r = tid, tid + 128, ...,
-  // finding a record's cell by binary search of the offsets; kept records are packed at the
-  // start of their cell's range
-  for (int rr = tid; rr < all; rr += kBlock) {
-    int lo = 0, hi = total - 1;  // last local cell with coff <= rr (and a record there)
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (coff[mid] <= rr) lo = mid; else hi = mid - 1;
+  // copy: each thread its own cells, four records requested together; a cell that lost records
+  // to its capacity keeps at most its count (the replica's build is replayed with larger cells)
+  float4 first[4][4];  // the first four records of each of the thread's cells, requested together
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int u = 0; u < 4; u++)  // in bounds for every k (an unmarked cell reads cell c0's first record)
+      first[k][u] = cells[(size_t)gcell[k] * cap + min(u, max(src[k] - 1, 0))];
+  int kept[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    int w = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int j = __float_as_int(first[k][u].w);
+      if (u < src[k] && (j < excl_lo || j >= excl_hi)) {
+        if (w < loc[k]) rec[off[k] + w] = first[k][u];
+        w++;
+      }
     }
-    while (cfull[lo] == 0 || coff[lo] + cfull[lo] <= rr) lo--;  // skip empty cells sharing the offset
-    const int gl = c0 + (ax[2].global(lo / (sx * sy)) * g + ax[1].global((lo / sx) % sy)) * g + ax[0].global(lo % sx);
-    const float4 v = cells[(size_t)gl * cap + (rr - coff[lo])];
-    const int j = __float_as_int(v.w);
-    if (j >= excl_lo && j < excl_hi) continue;
-    rec[coff[lo] + atomicAdd(&cnum[lo], 1)] = v;
+    kept[k] = w;
   }
+  // cells with more than four records (rare): the rest, cell by cell
+  if (src[0] > 4) copy_cell(cells + (size_t)gcell[0] * cap, src[0], loc[0], rec + off[0], kept[0], excl_lo, excl_hi);
+  if (src[1] > 4) copy_cell(cells + (size_t)gcell[1] * cap, src[1], loc[1], rec + off[1], kept[1], excl_lo, excl_hi);
+  if (src[2] > 4) copy_cell(cells + (size_t)gcell[2] * cap, src[2], loc[2], rec + off[2], kept[2], excl_lo, excl_hi);
+  if (src[3] > 4) copy_cell(cells + (size_t)gcell[3] * cap, src[3], loc[3], rec + off[3], kept[3], excl_lo, excl_hi);
   __syncthreads();
   return true;
 }
@@ -782,7 +819,7 @@ __device__ __forceinline__ void for_stencil(int c0, int g, int cx, int cy, int c
 // Lists of bead gi (active thread) straight from the global cells, written unsorted and then
 // sorted in place: the path of a workgroup whose neighbourhood does not fit in LDS.
 __device__ __forceinline__ void cell_query_direct(const Dev& d, const RepParams& P, int s, int gi, int tid, const LjTest T,
-                                  const int* ttype, const float* tsig, int* counts) {
+                                  const float2* tts, int* counts) {
   int cnt_lj = 0, cnt_q = 0;
   const int np = d.topo.np, nq = d.topo.nq;
   const float inv_np = 1.0f / (float)np;
@@ -797,7 +834,7 @@ __device__ __forceinline__ void cell_query_direct(const Dev& d, const RepParams&
         const float4 e = cell[m];
         const int j = __float_as_int(e.w);
         const int lj = local_of(j, np, inv_np);
-        if (j - lj == s * np || !T(e, ttype[lj], tsig[lj])) continue;
+        if (j - lj == s * np || !T(e, __float_as_int(tts[lj].x), tts[lj].y)) continue;
         if (cnt_lj < d.lj_cap) d.lj_list[(size_t)cnt_lj * d.Nb + gi] = j;
         cnt_lj++;
       }
@@ -836,19 +873,21 @@ __device__ __forceinline__ void cell_query_direct(const Dev& d, const RepParams&
 // each bead keeps its LJ hits insertion-sorted in LDS, the Yukawa hits are ranked by all
 // threads together. Neighbourhoods that do not fit in LDS take the direct path.
 __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild) {
+  // LDS shared between phases: the Yukawa slots reuse the records once the LJ lists are out,
+  // and the own-bead cell counts of the gather the LJ slots before the LJ scan
   __shared__ float4 rec[kCqRecords];
   __shared__ int cnum[kCqCells];
-  __shared__ int cfull[kCqCells];
   __shared__ int coff[kCqCells + 1];
   __shared__ CellAxis ax[3];
   __shared__ int red_i[16];
   __shared__ int ljbuf[kCqLjSlots * kBlock];
-  __shared__ int qbuf[kCqQMax * kCqQSlots];
+  int* const ownc = ljbuf;
+  int* const qbuf = reinterpret_cast<int*>(rec);
   __shared__ int qn[kCqQMax];
+  __shared__ int qoff[kCqQMax + 1];
   __shared__ float4 qsh[kCqQMax];
   __shared__ int qcell[kCqQMax][3];
-  __shared__ float tsig[kBlock];
-  __shared__ int ttype[kBlock];
+  __shared__ float2 tts[kBlock];  // (type bits, sigma) of each template bead, one load per record
   __shared__ unsigned long long tmask[kMaxTypes];
   __shared__ double red[4];
   const int s = blockIdx.x, tid = threadIdx.x;
@@ -859,7 +898,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   const int np = d.topo.np, nq = d.topo.nq;
   const bool active = tid < np;
   const int gi = s * np + tid;
-  if (active) { tsig[tid] = (float)d.topo.sigma[tid]; ttype[tid] = d.topo.type[tid]; }
+  if (active) tts[tid] = make_float2(__int_as_float(d.topo.type[tid]), (float)d.topo.sigma[tid]);
   if (tid < kMaxTypes) tmask[tid] = d.topo.att_mask[tid];
   if (tid < kCqQMax) qn[tid] = 0;
   const float box = (float)P.box, hbox = (float)P.hbox;
@@ -881,47 +920,63 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   int rel[3];
   // ---- LJ / WCA: the neighbourhood's records of other subunits, gathered into LDS
   const bool lj_ok = gather_cells(d.cell_lj, d.cnt_lj, d.ccap_lj, P.c0_lj, P.g_lj, P.inv_a_lj, hbox, cx, cy, cz, active,
-                                  T.ix, T.iy, T.iz, tid, ax, rel, cnum, cfull, coff, rec, red_i, s * np, (s + 1) * np);
+                                  T.ix, T.iy, T.iz, tid, ax, rel, cnum, ownc, coff, rec, red_i, s * np, (s + 1) * np);
   bool direct = !lj_ok;
   if (lj_ok && active) {
-    // the local box in registers
+    // the local box in registers; with no wrap along x the three cells of a (dz, dy) row are one
+    // contiguous record range, otherwise each of the 27 cells is its own range
     const int g = ax[0].g, sx = ax[0].span, sy = ax[1].span;
     const bool fx = ax[0].full, fy = ax[1].full, fz = ax[2].full;
     const int ox = fx ? ax[0].cc : -ax[0].lo, oy = fy ? ax[1].cc : -ax[1].lo, oz = fz ? ax[2].cc : -ax[2].lo;
-    const int d0 = g == 1 ? 0 : -1, d1 = g == 1 ? 0 : 1;
+    const bool rows = !fx && g > 1;
+    const int nseg = g == 1 ? 1 : (rows ? 9 : 27);
     bool over = false;
     for (int pass = 0; pass < 2; pass++) {  // pass 1 only when the LDS slots overflowed
       if (pass == 1) {
         if (!over) break;
         cnt_lj = 0;
       }
-      for (int dz = d0; dz <= d1; dz++) {
-        const int lz = fz ? wrap_cell(oz + rel[2] + dz, g) : rel[2] + dz + oz;
-        for (int dy = d0; dy <= d1; dy++) {
+      for (int sg = 0; sg < nseg; sg++) {
+        int m0, m1;
+        if (g == 1) {
+          m0 = coff[0]; m1 = m0 + cnum[0];
+        } else {
+          const int dz = rows ? sg / 3 - 1 : sg / 9 - 1, dy = rows ? sg % 3 - 1 : (sg / 3) % 3 - 1;
+          const int lz = fz ? wrap_cell(oz + rel[2] + dz, g) : rel[2] + dz + oz;
           const int ly = fy ? wrap_cell(oy + rel[1] + dy, g) : rel[1] + dy + oy;
-          for (int dx = d0; dx <= d1; dx++) {
-            const int lxx = fx ? wrap_cell(ox + rel[0] + dx, g) : rel[0] + dx + ox;
-            const int lc = (lz * sy + ly) * sx + lxx;
-            const int m0 = coff[lc], m1 = m0 + cnum[lc];
-            for (int m = m0; m < m1; m++) {
-              const float4 e = rec[m];
-              const int j = __float_as_int(e.w);
-              const int lj = local_of(j, np, inv_np);
-              if (!T(e, ttype[lj], tsig[lj])) continue;
-              if (pass == 1) {
-                if (cnt_lj < d.lj_cap) d.lj_list[(size_t)cnt_lj * d.Nb + gi] = j;
-              } else if (cnt_lj < kCqLjSlots) {
-                int p = cnt_lj;
-                while (p > 0 && ljbuf[(p - 1) * kBlock + tid] > j) {
-                  ljbuf[p * kBlock + tid] = ljbuf[(p - 1) * kBlock + tid];
-                  p--;
-                }
-                ljbuf[p * kBlock + tid] = j;
-              } else {
-                over = true;
+          if (rows) {
+            const int l0 = (lz * sy + ly) * sx + rel[0] - 1 + ox;
+            m0 = coff[l0]; m1 = coff[l0 + 3];
+          } else {
+            const int lc = (lz * sy + ly) * sx + wrap_cell(ox + rel[0] + sg % 3 - 1, g);
+            m0 = coff[lc]; m1 = m0 + cnum[lc];
+          }
+        }
+        for (int m = m0; m < m1; m += 4) {  // four records and their template data requested together
+          float4 e[4];
+          float2 tj[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) e[u] = rec[min(m + u, m1 - 1)];
+#pragma unroll
+          for (int u = 0; u < 4; u++) tj[u] = tts[local_of(__float_as_int(e[u].w), np, inv_np)];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (m + u >= m1) break;
+            const int j = __float_as_int(e[u].w);
+            if (!T(e[u], __float_as_int(tj[u].x), tj[u].y)) continue;
+            if (pass == 1) {
+              if (cnt_lj < d.lj_cap) d.lj_list[(size_t)cnt_lj * d.Nb + gi] = j;
+            } else if (cnt_lj < kCqLjSlots) {
+              int p = cnt_lj;
+              while (p > 0 && ljbuf[(p - 1) * kBlock + tid] > j) {
+                ljbuf[p * kBlock + tid] = ljbuf[(p - 1) * kBlock + tid];
+                p--;
               }
-              cnt_lj++;
+              ljbuf[p * kBlock + tid] = j;
+            } else {
+              over = true;
             }
+            cnt_lj++;
           }
         }
       }
@@ -978,18 +1033,33 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
       }
     }
     __syncthreads();
-    // rank sort of each list: an entry's rank is the number of smaller ids in it
-    for (int a = 0; a < nql; a++) {
-      const int n = qn[a];
-      if (n > kCqQSlots) continue;  // overflowed: its bead takes the direct path below
-      const int gq = s * nq + a;
-      const int* buf = qbuf + a * kCqQSlots;
-      for (int k = tid; k < n; k += kBlock) {
-        const int v = buf[k];
-        int rank = 0;
-        for (int m = 0; m < n; m++) rank += buf[m] < v ? 1 : 0;
-        if (rank < d.q_cap) d.q_list[(size_t)rank * d.Nq + gq] = v;
+    // rank sort: an entry's rank is the number of smaller ids in its list; the entries of all
+    // lists are spread over the threads (lists that overflowed their slots are left to the
+    // direct path below)
+    if (tid == 0) {
+      int o = 0;
+      for (int a = 0; a < nql; a++) {
+        qoff[a] = o;
+        o += qn[a] <= kCqQSlots ? qn[a] : 0;
       }
+      qoff[nql] = o;
+    }
+    __syncthreads();
+    for (int e = tid; e < qoff[nql]; e += kBlock) {
+      int a = 0;
+      while (a + 1 < nql && qoff[a + 1] <= e) a++;
+      const int n = qn[a];
+      const int* buf = qbuf + a * kCqQSlots;
+      const int v = buf[e - qoff[a]];
+      const int4* b4 = reinterpret_cast<const int4*>(buf);
+      int rank = 0;
+#pragma unroll 4
+      for (int m = 0; m < (n >> 2); m++) {
+        const int4 w = b4[m];
+        rank += (w.x < v) + (w.y < v) + (w.z < v) + (w.w < v);
+      }
+      for (int m = n & ~3; m < n; m++) rank += buf[m] < v ? 1 : 0;
+      if (rank < d.q_cap) d.q_list[(size_t)rank * d.Nq + s * nq + a] = v;
     }
     if (qi >= 0) {
       const int gq = s * nq + qi;
@@ -1021,7 +1091,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
     __syncthreads();
     if (active) {
       int* cts = ljbuf + 2 * tid;  // two LDS words per thread, free by now
-      cell_query_direct(d, P, s, gi, tid, T, ttype, tsig, cts);
+      cell_query_direct(d, P, s, gi, tid, T, tts, cts);
       cnt_lj = cts[0];
       cnt_q = cts[1];
       const int qi = d.topo.qslot[tid];
@@ -1041,8 +1111,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   if ((tid & 63) == 0) { red[2 * (tid >> 6)] = wl; red[2 * (tid >> 6) + 1] = wq; }
   __syncthreads();
   if (tid == 0) {
-    d.part[(size_t)s * kPartStride + 7] = red[0] + red[2];
-    d.part[(size_t)s * kPartStride + 8] = red[1] + red[3];
+    d.part[7 * (size_t)d.Ns + s] = red[0] + red[2];
+    d.part[8 * (size_t)d.Ns + s] = red[1] + red[3];
   }
 }
 
@@ -1125,10 +1195,11 @@ __device__ __forceinline__ PT yk_term(PT r2, PT qq, PT yk, PT kappa, PT ecut_el2
   return (qq * exp(-kappa * rr) / r2) * (kappa + (PT)1 / rr) * yk;
 }
 
-// LDS bytes of k_force: own positions, hinge forces, Yukawa tables of the charged beads, and
-// the pair terms of one gather round of `tile` entries.
+// LDS bytes of k_force: own positions, the hinge forces (three beads per hinge; the fourth is
+// minus their sum) sharing their space with the pair terms of one gather round of `tile`
+// entries (the bending sums are taken before the pair rounds), and the Yukawa tables.
 static inline size_t force_lds_bytes(int np, int nh, int nq, int tile) {
-  return (size_t)np * 32 + (size_t)nh * 96 + (size_t)nq * nq * 16 + (size_t)tile * 3 * 8;
+  return (size_t)np * 32 + (size_t)std::max(nh * 9, tile * 3) * 8 + (size_t)nq * nq * 16;
 }
 
 // kChunk: list entries the workgroup evaluates per gather round (the tile).
@@ -1142,10 +1213,10 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
   __shared__ unsigned char owner_s[kChunk];
   const int np = d.topo.np, nh = d.topo.nh, nq = d.topo.nq;
   double4* own = reinterpret_cast<double4*>(smem);
-  double* hc = reinterpret_cast<double*>(own + np);
-  double* qq_s = hc + 12 * nh;
+  double* hc = reinterpret_cast<double*>(own + np);          // [nh][9] forces on hinge beads 1-3
+  PT* pf = reinterpret_cast<PT*>(hc);                        // [3][kChunk], after the bending sums
+  double* qq_s = hc + max(9 * nh, 3 * kChunk);
   double* yk_s = qq_s + nq * nq;
-  PT* pf = reinterpret_cast<PT*>(yk_s + nq * nq);  // [3][kChunk]
 
   const int s = blockIdx.x, tid = threadIdx.x;
   const int r = d.sub_rep[s];
@@ -1204,7 +1275,10 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
   // bending hinges -> LDS
   for (int h = tid; h < nh; h += kBlock) {
     const int4 hg = (h == tid) ? hg0 : reinterpret_cast<const int4*>(d.topo.hinge)[h];
-    hinge_forces(own, hg, box, hbox, kb, hc + 12 * h);
+    double hf[12];
+    hinge_forces(own, hg, box, hbox, kb, hf);
+#pragma unroll
+    for (int k = 0; k < 9; k++) hc[9 * h + k] = hf[3 + k];
   }
   // stretching (src/bead.cpp:17-23) over the bead's edge row
   double sx = 0, sy = 0, sz = 0;
@@ -1222,6 +1296,25 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
       const double L = sqrt(dx * dx + dy * dy + dz * dz);
       const double c = (l0 - L) * ks / L;
       sx += dx * c; sy += dy * c; sz += dz * c;
+    }
+  }
+  // bending gather (fixed hinge order); hinge bead 0 takes minus the sum of the other three,
+  // formed as the hinge computes it
+  __syncthreads();
+  double bx = 0, by = 0, bz = 0;
+  if (active) {
+    const int hw[8] = {hrow0.x, hrow0.y, hrow0.z, hrow0.w, hrow1.x, hrow1.y, hrow1.z, hrow1.w};
+#pragma unroll
+    for (int k = 0; k < kMaxHinc; k++) {
+      if (k >= hcnt) break;
+      const int sl = (hw[k >> 1] >> (16 * (k & 1))) & 0xffff;
+      const double* q = hc + 9 * (sl >> 2);
+      if ((sl & 3) == 0) {
+        bx += -(q[0] + q[3] + q[6]); by += -(q[1] + q[4] + q[7]); bz += -(q[2] + q[5] + q[8]);
+      } else {
+        const double* f = q + 3 * ((sl & 3) - 1);
+        bx += f[0]; by += f[1]; bz += f[2];
+      }
     }
   }
   // pair terms: all threads evaluate the subunit's list entries kChunk at a time; each bead
@@ -1283,16 +1376,6 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
 #pragma unroll
   for (int k = 0; k < kPartStride; k++) part[k] = 0.0;
   if (active) {
-    // bending gather (fixed hinge order)
-    double bx = 0, by = 0, bz = 0;
-    const int hw[8] = {hrow0.x, hrow0.y, hrow0.z, hrow0.w, hrow1.x, hrow1.y, hrow1.z, hrow1.w};
-#pragma unroll
-    for (int k = 0; k < kMaxHinc; k++) {
-      if (k >= hcnt) break;
-      const int sl = (hw[k >> 1] >> (16 * (k & 1))) & 0xffff;
-      const double* q = hc + 12 * (sl >> 2) + 3 * (sl & 3);
-      bx += q[0]; by += q[1]; bz += q[2];
-    }
     // src/forces.cpp:141-144: sforce + bforce + (eForce + ljForce); in split mode eForce is
     // the near-field list sum plus the far field of the charged bead
     double exd = (double)ex, eyd = (double)ey, ezd = (double)ez;
@@ -1346,7 +1429,7 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
     if ((tid & 63) == 0) red[tid >> 6][k] = w;
   }
   __syncthreads();
-  if (tid < 7) d.part[(size_t)s * kPartStride + tid] = red[0][tid] + red[1][tid];
+  if (tid < 7) d.part[tid * (size_t)d.Ns + s] = red[0][tid] + red[1][tid];
 }
 
 // One workgroup per replica: fixed-order sums of the per-subunit partials, then the scalar
@@ -1359,43 +1442,36 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
   const int r = blockIdx.x, tid = threadIdx.x;
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
+  // slot-major partials: a wave reads 64 consecutive subunits of one slot per load; each thread
+  // adds its subunits tid, tid + kFinThreads, ... in that order
   double acc[kPartStride];
 #pragma unroll
   for (int k = 0; k < kPartStride; k++) acc[k] = 0.0;
   const int s_end = P.s0 + P.ns;
-  for (int s0 = P.s0 + tid; s0 < s_end; s0 += 2 * kFinThreads) {
-    double v[2][kPartStride];
+  const size_t ns_all = d.Ns;
+  for (int s = P.s0 + tid; s < s_end; s += kFinThreads) {
+    double v[kPartSlots];
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int s = s0 + u * kFinThreads;
-      const double2* src = reinterpret_cast<const double2*>(d.part + (size_t)s * kPartStride);
+    for (int k = 0; k < kPartSlots; k++) v[k] = d.part[k * ns_all + s];
 #pragma unroll
-      for (int k = 0; k < kPartStride / 2; k++) {
-        const double2 w = (s < s_end) ? src[k] : make_double2(0.0, 0.0);
-        v[u][2 * k] = w.x; v[u][2 * k + 1] = w.y;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 2; u++)
-#pragma unroll
-      for (int k = 0; k < kPartStride; k++) acc[k] += v[u][k];
+    for (int k = 0; k < kPartSlots; k++) acc[k] += v[k];
   }
 #pragma unroll
-  for (int k = 0; k < kPartStride; k++) {
+  for (int k = 0; k < kPartSlots; k++) {
     const double w = wave_sum(acc[k]);
     if ((tid & 63) == 0) wred[tid >> 6][k] = w;
   }
   __syncthreads();
-  if (tid < kPartStride) {
+  if (tid < kPartSlots) {
     double t = 0.0;
     for (int w = 0; w < kFinThreads / 64; w++) t += wred[w][tid];
     tot[tid] = t;
   }
   __syncthreads();
   if (tid != 0) return;
-  double red[kPartStride][1];
+  double red[kPartSlots][1];
 #pragma unroll
-  for (int k = 0; k < kPartStride; k++) red[k][0] = tot[k];
+  for (int k = 0; k < kPartSlots; k++) red[k][0] = tot[k];
   const double ke = red[0][0];
   const int q = P.q_chain;
   const double dt = P.dt, T = P.T;

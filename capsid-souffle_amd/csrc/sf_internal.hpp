@@ -47,7 +47,8 @@ constexpr int kBlock = 128;      // threads of a subunit workgroup; templates ne
 constexpr int kMaxChain = 8;     // Nose-Hoover chain length limit
 constexpr int kMaxTypes = 64;    // bead type ids must be < kMaxTypes
 constexpr int kMaxCand = 1024;   // candidate subunits per subunit at a list build
-constexpr int kPartStride = 10;  // per-subunit partial sums
+constexpr int kPartStride = 10;  // per-subunit partial sums (slot-major: part[k * Ns + s])
+constexpr int kPartSlots = 9;    // slots in use: KE, sum v (3), BD noise (3), LJ and Yukawa list sizes
 constexpr double kWcaFactor = 1.12246205;  // src/forces.cpp:27, src/energies.cpp:67
 constexpr uint32_t kLcgMod = 2147483647u;  // minstd_rand0: x <- 16807 x mod (2^31 - 1)
 constexpr uint32_t kLcgMul = 16807u;
@@ -165,7 +166,7 @@ struct Dev {
   double4* centres;    // [Ns] centre-bead positions
   int* lj_list; int* lj_cnt;
   int* q_list; int* q_cnt;
-  double* part;        // [Ns][kPartStride]
+  double* part;        // [kPartStride][Ns]
   double* epart;       // [Ns][5] energy partial sums
   double* ebead;       // [Nb][5] per-bead energies
   double* out_f; double* out_s; double* out_b; double* out_e; double* out_lj;  // [Nb][3], nullable
