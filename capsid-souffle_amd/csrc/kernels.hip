@@ -31,6 +31,15 @@ __device__ __forceinline__ float min_imgf(float v, float box, float hbox) {
   return v;
 }
 
+// Template index of bead j (subunit-major numbering) by a float reciprocal and an exact fix-up.
+__device__ __forceinline__ int local_of(int j, int np, float inv_np) {
+  int q = (int)((float)j * inv_np);
+  int r = j - q * np;
+  while (r < 0) r += np;
+  while (r >= np) r -= np;
+  return r;
+}
+
 __device__ __forceinline__ int tag_sub(double w) { return (int)(__double_as_longlong(w) >> 32); }
 __device__ __forceinline__ int tag_loc(double w) { return (int)(__double_as_longlong(w) & 0xffffffffll); }
 
@@ -115,9 +124,13 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_integrate(Dev d) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= d.Nb) return;
-  double4 p = d.pos[b];
-  const int s = tag_sub(p.w), l = tag_loc(p.w);
+  // subunit and template index from the bead index (beads are subunit-major), so the replica
+  // lookup does not wait for the position load
+  const int np = d.topo.np;
+  const int l = local_of(b, np, 1.0f / (float)np);
+  const int s = (int)((float)(b - l) * (1.0f / (float)np) + 0.5f);
   const int r = d.sub_rep[s];
+  double4 p = d.pos[b];
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
   const double box = P.box, hbox = P.hbox, dt = P.dt, half_skin2 = P.half_skin2;
@@ -286,13 +299,6 @@ __device__ void build_lists_exact(const Dev& d, const RepParams& P, int s, int t
 }
 
 // j mod np and j / np for j < 2^23 via a float reciprocal and one correction.
-__device__ __forceinline__ int local_of(int j, int np, float inv_np) {
-  int q = (int)((float)j * inv_np);
-  int r = j - q * np;
-  if (r < 0) r += np;
-  else if (r >= np) r -= np;
-  return r;
-}
 
 // Fast build. Candidate beads are taken relative to the centre bead of s (minimum image,
 // then FP32). LJ candidates inside the cube |rel| <= r_bound + rl_lj are counted and placed
@@ -606,7 +612,7 @@ constexpr int kCqLjSlots = 12;    // LDS slots per bead for its sorted LJ list
 constexpr int kCqQMax = 8;        // charged beads per template with LDS Yukawa slots
 constexpr int kCqQSlots = 128;    // LDS slots per charged bead for its Yukawa list (a multiple of 4)
 static_assert(kCqQMax * kCqQSlots * 4 <= kCqRecords * 16, "Yukawa slots live in the record space");
-static_assert(kCqCells <= kCqLjSlots * kBlock, "own-bead cell counts live in the LJ slots");
+static_assert(3 * kCqCells <= kCqLjSlots * kBlock, "the gather's per-cell tables live in the LJ slots");
 static_assert(kCqCells == 4 * kBlock, "the gather gives each thread four cells");
 
 // In-place insertion sort of one list column (stride n) in global memory; used only when a
@@ -637,13 +643,21 @@ __device__ __forceinline__ int rel_cell(int c, int cc, int g) {
 struct LjTest {
   float ix, iy, iz, si, ratt2, skinf, box, hbox;
   unsigned long long mask_i;
-  __device__ __forceinline__ bool operator()(const float4& e, int tj, float sj) const {
+  __device__ __forceinline__ float r2_to(const float4& e) const {
     const float dx = min_imgf(ix - e.x, box, hbox), dy = min_imgf(iy - e.y, box, hbox), dz = min_imgf(iz - e.z, box, hbox);
-    const float r2 = dx * dx + dy * dy + dz * dz;
+    return dx * dx + dy * dy + dz * dz;
+  }
+  __device__ __forceinline__ bool hit(float r2, int tj, float sj) const {
     if ((mask_i >> tj) & 1ull) return r2 < ratt2;
     const float w = (float)kWcaFactor * 0.5f * (si + sj) + skinf;
     return r2 < w * w;
   }
+  // the largest list radius^2 of this bead over partners of sigma <= smax (a bound for hit())
+  __device__ __forceinline__ float bound2(float smax) const {
+    const float w = (float)kWcaFactor * 0.5f * (si + smax) + skinf;
+    return mask_i ? fmaxf(ratt2, w * w) : w * w;
+  }
+  __device__ __forceinline__ bool operator()(const float4& e, int tj, float sj) const { return hit(r2_to(e), tj, sj); }
 };
 
 // Local box of cells around a subunit on one axis: cells cc + lo .. cc + lo + span - 1
@@ -654,25 +668,6 @@ struct CellAxis {
   __device__ __forceinline__ int local(int rel) const { return full ? wrap_cell(cc + rel, g) : rel - lo; }
   __device__ __forceinline__ int global(int l) const { return full ? l : wrap_cell(cc + lo + l, g); }
 };
-
-// Records 4.. of one cell (sn stored) whose ids are outside [excl_lo, excl_hi), appended to
-// dst from position w (the kept ones among the first four), at most n in all, in stored order.
-__device__ __forceinline__ void copy_cell(const float4* cp, int sn, int n, float4* dst, int w, int excl_lo,
-                                          int excl_hi) {
-  for (int m = 4; m < sn; m += 4) {
-    float4 v[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) v[u] = cp[min(m + u, sn - 1)];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int j = __float_as_int(v[u].w);
-      if (m + u < sn && (j < excl_lo || j >= excl_hi)) {
-        if (w < n) dst[w] = v[u];
-        w++;
-      }
-    }
-  }
-}
 
 // The cells of a g^3 grid (first cell c0) that the 27-cell stencils of a workgroup's
 // participating beads touch are marked, and the records of other subunits in them (the cell's
@@ -736,8 +731,9 @@ __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt
     atomicAdd(&ownc[(ax[2].local(rz) * sy + ax[1].local(ry)) * sx + ax[0].local(rx)], 1);
   }
   __syncthreads();
-  // records of other subunits per marked cell, then an exclusive prefix sum
-  int loc[4], src[4], gcell[4], sum = 0;
+  // records of other subunits per marked cell, then exclusive prefix sums of those (the packed
+  // LDS layout) and of all stored records (the copy's work list)
+  int loc[4], src[4], gcell[4], sum = 0, ssum = 0;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int lc = 4 * tid + k;
@@ -748,56 +744,54 @@ __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt
       n = max(sn - ownc[lc], 0);
     }
     loc[k] = n; src[k] = sn; gcell[k] = gl;
-    sum += n;
+    sum += n; ssum += sn;
   }
-  int incl = sum;
+  int incl = sum, sincl = ssum;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const int v = __shfl_up(incl, o, 64);
-    if ((tid & 63) >= o) incl += v;
+    const int v = __shfl_up(incl, o, 64), sv = __shfl_up(sincl, o, 64);
+    if ((tid & 63) >= o) { incl += v; sincl += sv; }
   }
   __syncthreads();
-  if ((tid & 63) == 63) red_i[tid >> 6] = incl;
+  if ((tid & 63) == 63) { red_i[tid >> 6] = incl; red_i[2 + (tid >> 6)] = sincl; }
   __syncthreads();
   const int base = incl - sum + ((tid >> 6) ? red_i[0] : 0);
-  const int all = red_i[0] + red_i[1];
-  int off[4], acc = base;
+  const int sbase = sincl - ssum + ((tid >> 6) ? red_i[2] : 0);
+  const int all = red_i[0] + red_i[1], all_src = red_i[2] + red_i[3];
+  // the own-bead counts are no longer needed: their space holds, per local cell, the end of its
+  // stored records in the work list, its global cell and a running count of kept records
+  int* send = ownc;
+  int* gcl = ownc + kCqCells;
+  int* run = ownc + 2 * kCqCells;
+  int acc = base, sacc = sbase;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int lc = 4 * tid + k;
-    off[k] = acc;
-    if (lc < kCqCells) { coff[lc] = acc; cnum[lc] = loc[k]; }
+    coff[lc] = acc;
+    cnum[lc] = loc[k];
     acc += loc[k];
+    sacc += src[k];
+    send[lc] = sacc;
+    gcl[lc] = gcell[k];
+    run[lc] = 0;
   }
   if (tid == 0) coff[kCqCells] = all;
   if (all > kCqRecords) return false;  // uniform
-  // copy: each thread its own cells, four records requested together; a cell that lost records
-  // to its capacity keeps at most its count (the replica's build is replayed with larger cells)
-  float4 first[4][4];  // the first four records of each of the thread's cells, requested together
-#pragma unroll
-  for (int k = 0; k < 4; k++)
-#pragma unroll
-    for (int u = 0; u < 4; u++)  // in bounds for every k (an unmarked cell reads cell c0's first record)
-      first[k][u] = cells[(size_t)gcell[k] * cap + min(u, max(src[k] - 1, 0))];
-  int kept[4];
-#pragma unroll
-  for (int k = 0; k < 4; k++) {
-    int w = 0;
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int j = __float_as_int(first[k][u].w);
-      if (u < src[k] && (j < excl_lo || j >= excl_hi)) {
-        if (w < loc[k]) rec[off[k] + w] = first[k][u];
-        w++;
-      }
+  __syncthreads();
+  // copy, one stored record per thread and round: its cell is the first with send > rr
+  for (int rr = tid; rr < all_src; rr += kBlock) {
+    int lo = 0, hi = kCqCells - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (send[mid] > rr) hi = mid; else lo = mid + 1;
     }
-    kept[k] = w;
+    const int start = lo ? send[lo - 1] : 0;
+    const float4 v = cells[(size_t)gcl[lo] * cap + (rr - start)];
+    const int j = __float_as_int(v.w);
+    if (j >= excl_lo && j < excl_hi) continue;
+    const int w = atomicAdd(&run[lo], 1);
+    if (w < cnum[lo]) rec[coff[lo] + w] = v;
   }
-  // cells with more than four records (rare): the rest, cell by cell
-  if (src[0] > 4) copy_cell(cells + (size_t)gcell[0] * cap, src[0], loc[0], rec + off[0], kept[0], excl_lo, excl_hi);
-  if (src[1] > 4) copy_cell(cells + (size_t)gcell[1] * cap, src[1], loc[1], rec + off[1], kept[1], excl_lo, excl_hi);
-  if (src[2] > 4) copy_cell(cells + (size_t)gcell[2] * cap, src[2], loc[2], rec + off[2], kept[2], excl_lo, excl_hi);
-  if (src[3] > 4) copy_cell(cells + (size_t)gcell[3] * cap, src[3], loc[3], rec + off[3], kept[3], excl_lo, excl_hi);
   __syncthreads();
   return true;
 }
@@ -888,6 +882,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   __shared__ float4 qsh[kCqQMax];
   __shared__ int qcell[kCqQMax][3];
   __shared__ float2 tts[kBlock];  // (type bits, sigma) of each template bead, one load per record
+  __shared__ float smax_w[2];
   __shared__ unsigned long long tmask[kMaxTypes];
   __shared__ double red[4];
   const int s = blockIdx.x, tid = threadIdx.x;
@@ -915,7 +910,14 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   const float ratt = (float)(P.ecut + P.skin) + 1e-3f;
   T.ratt2 = ratt * ratt;
   T.skinf = (float)P.skin + 1e-3f;
+  {  // largest template sigma, for the cheap distance filter of the LJ scan
+    float sm = T.si;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sm = fmaxf(sm, __shfl_xor(sm, o, 64));
+    if ((tid & 63) == 0) smax_w[tid >> 6] = sm;
+  }
   __syncthreads();
+  const float bound2 = T.bound2(fmaxf(smax_w[0], smax_w[1]));
   int cnt_lj = 0, cnt_q = 0;
   int rel[3];
   // ---- LJ / WCA: the neighbourhood's records of other subunits, gathered into LDS
@@ -952,18 +954,19 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
             m0 = coff[lc]; m1 = m0 + cnum[lc];
           }
         }
-        for (int m = m0; m < m1; m += 4) {  // four records and their template data requested together
+        for (int m = m0; m < m1; m += 4) {  // four records requested together
           float4 e[4];
-          float2 tj[4];
 #pragma unroll
           for (int u = 0; u < 4; u++) e[u] = rec[min(m + u, m1 - 1)];
 #pragma unroll
-          for (int u = 0; u < 4; u++) tj[u] = tts[local_of(__float_as_int(e[u].w), np, inv_np)];
-#pragma unroll
           for (int u = 0; u < 4; u++) {
             if (m + u >= m1) break;
+            // distance first; the pair's type and sigma only for records inside the bound
+            const float r2 = T.r2_to(e[u]);
+            if (!(r2 < bound2)) continue;
             const int j = __float_as_int(e[u].w);
-            if (!T(e[u], __float_as_int(tj[u].x), tj[u].y)) continue;
+            const float2 tj = tts[local_of(j, np, inv_np)];
+            if (!T.hit(r2, __float_as_int(tj.x), tj.y)) continue;
             if (pass == 1) {
               if (cnt_lj < d.lj_cap) d.lj_list[(size_t)cnt_lj * d.Nb + gi] = j;
             } else if (cnt_lj < kCqLjSlots) {
@@ -1437,22 +1440,31 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
 constexpr int kFinThreads = 1024;
 
 __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
-  __shared__ double wred[kFinThreads / 64][kPartStride];
-  __shared__ double tot[kPartStride];
+  __shared__ double wred[kFinThreads / 64][kPartSlots];
+  __shared__ double tot[kPartSlots];
   const int r = blockIdx.x, tid = threadIdx.x;
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
+  // wave 0's lanes take the chain state now (lane j: bath j), so the loads overlap the sums
+  const int q = P.q_chain, brownian = P.brownian;
+  const bool lists = init || S.rebuild;
+  double xi_l = 0.0;
+  if (tid < kMaxChain && !init && !brownian) xi_l = S.xi_n[tid];
   // slot-major partials: a wave reads 64 consecutive subunits of one slot per load; each thread
-  // adds its subunits tid, tid + kFinThreads, ... in that order
-  double acc[kPartStride];
+  // adds its subunits tid, tid + kFinThreads, ... in that order. Slots 4-6 (BD noise) and 7-8
+  // (list sizes, written at list builds) are read only when they are used.
+  double acc[kPartSlots];
 #pragma unroll
-  for (int k = 0; k < kPartStride; k++) acc[k] = 0.0;
+  for (int k = 0; k < kPartSlots; k++) acc[k] = 0.0;
   const int s_end = P.s0 + P.ns;
   const size_t ns_all = d.Ns;
   for (int s = P.s0 + tid; s < s_end; s += kFinThreads) {
     double v[kPartSlots];
 #pragma unroll
-    for (int k = 0; k < kPartSlots; k++) v[k] = d.part[k * ns_all + s];
+    for (int k = 0; k < kPartSlots; k++) {
+      const bool used = k < 4 || (k < 7 ? brownian != 0 : lists);
+      v[k] = used ? d.part[k * ns_all + s] : 0.0;
+    }
 #pragma unroll
     for (int k = 0; k < kPartSlots; k++) acc[k] += v[k];
   }
@@ -1468,15 +1480,21 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
     tot[tid] = t;
   }
   __syncthreads();
-  if (tid != 0) return;
-  double red[kPartSlots][1];
-#pragma unroll
-  for (int k = 0; k < kPartSlots; k++) red[k][0] = tot[k];
-  const double ke = red[0][0];
-  const int q = P.q_chain;
+  if (tid >= 64) return;
+  const double ke = tot[0];
   const double dt = P.dt, T = P.T;
-  if (!P.brownian) {
-    // the chain lives in registers for the scalar recurrence (loops unrolled over kMaxChain)
+  if (!brownian) {
+    // forward chain of this step (src/functions.cpp:33-58 in bath order): bath j's update
+    // reads xi[j+1] before that bath is updated, so lane j + 1 evaluates both exponentials of
+    // bath j from its own xi at once
+    const double e1 = exp(-0.5 * dt * xi_l), e2 = exp(-0.25 * dt * xi_l);
+    double E1[kMaxChain], E2[kMaxChain];
+#pragma unroll
+    for (int j = 0; j < kMaxChain; j++) {
+      E1[j] = __shfl(e1, j + 1, 64);
+      E2[j] = __shfl(e2, j + 1, 64);
+    }
+    if (tid != 0) return;
     double Qb[kMaxChain], dof[kMaxChain], xi[kMaxChain + 1], eta[kMaxChain];
 #pragma unroll
     for (int j = 0; j < kMaxChain; j++) {
@@ -1491,8 +1509,11 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
       for (int j = 0; j < kMaxChain; j++)
         if (j < q && Qb[j] != 0) eta[j] = eta[j] + 0.5 * dt * xi[j];
 #pragma unroll
-      for (int j = 0; j < kMaxChain; j++)
-        if (j < q) chain_xi(j, xi, Qb, dof, T, dt, ke);
+      for (int j = 0; j < kMaxChain; j++) {
+        if (j >= q || Qb[j] == 0) continue;
+        const double g = j == 0 ? 2 * ke - dof[0] * 1.0 * T : Qb[j - 1] * xi[j - 1] * xi[j - 1] - dof[j] * 1.0 * T;
+        xi[j] = xi[j] * E1[j] + 0.5 * dt * (1.0 / Qb[j]) * g * E2[j];
+      }
     }
 #pragma unroll
     for (int j = 0; j < kMaxChain; j++) { S.xi[j] = xi[j]; S.eta[j] = eta[j]; }
@@ -1509,23 +1530,24 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
     S.expfac = ef;
     S.kick_c = 0.5 * dt * sqrt(ef);
   }
+  if (tid != 0) return;
   S.ke = ke;
   if (init) {
     S.mean_v[0] = S.mean_v[1] = S.mean_v[2] = 0.0;
     S.vcom = 0.0;
   } else {
-    S.mean_v[0] = red[1][0] * P.inv_nb; S.mean_v[1] = red[2][0] * P.inv_nb; S.mean_v[2] = red[3][0] * P.inv_nb;
+    S.mean_v[0] = tot[1] * P.inv_nb; S.mean_v[1] = tot[2] * P.inv_nb; S.mean_v[2] = tot[3] * P.inv_nb;
     S.vcom = sqrt(S.mean_v[0] * S.mean_v[0] + S.mean_v[1] * S.mean_v[1] + S.mean_v[2] * S.mean_v[2]);
     S.step += 1;
   }
-  if (P.brownian) {
+  if (brownian) {
     S.rng = init ? P.seed : lcg_mulmod(S.rng, P.mult6n);
-    S.mean_fran[0] = red[4][0] * P.inv_nb; S.mean_fran[1] = red[5][0] * P.inv_nb; S.mean_fran[2] = red[6][0] * P.inv_nb;
+    S.mean_fran[0] = tot[4] * P.inv_nb; S.mean_fran[1] = tot[5] * P.inv_nb; S.mean_fran[2] = tot[6] * P.inv_nb;
   }
-  if (init || S.rebuild) {
+  if (lists) {
     if (!init) S.rebuilds += 1;
-    S.lj_entries = (long long)red[7][0];
-    S.q_entries = (long long)red[8][0];
+    S.lj_entries = (long long)tot[7];
+    S.q_entries = (long long)tot[8];
   }
   S.rebuild = 0;
 }
