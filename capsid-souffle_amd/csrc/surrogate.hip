@@ -289,74 +289,97 @@ __device__ __forceinline__ unsigned short bf16_bits(float x) {
   return (unsigned short)(u >> 16);
 }
 
+// Hidden-layer input order of the packed W2, W3, W4: the inference keeps each layer's outputs
+// in the MFMA accumulator layout, where lane group g (lanes 16 g .. 16 g + 15) holds outputs
+// 16 t + 4 g .. 16 t + 4 g + 3 of every 16-row tile t. The next layer's K-block kb (32 inputs)
+// takes tiles 2 kb and 2 kb + 1 from the same lanes, so packed input position 32 kb + 8 g + j
+// is hidden unit 32 kb + 4 g + j (j < 4) or 32 kb + 16 + 4 g + j - 4 (j >= 4).
+__device__ __forceinline__ int hidden_of_position(int k) {
+  const int kb = k >> 5, g = (k >> 3) & 3, j = k & 7;
+  return 32 * kb + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
+}
+
 __global__ void k_pack_weights(const float* p, unsigned short* w, float* bias) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kFHid * kFLd1) {  // W1 [128][40], inputs 13..39 zero
     const int o = i / kFLd1, k = i - o * kFLd1;
     w[kFPackW1 + i] = bf16_bits(k < kFIn ? p[kFW1 + o * kFIn + k] : 0.0f);
   }
-  if (i < kFHid * kFLdH) {  // W2, W3 [128][136]
+  if (i < kFHid * kFLdH) {  // W2, W3 [128][136], inputs in the accumulator order
     const int o = i / kFLdH, k = i - o * kFLdH;
-    w[kFPackW2 + i] = bf16_bits(k < kFHid ? p[kFW2 + o * kFHid + k] : 0.0f);
-    w[kFPackW3 + i] = bf16_bits(k < kFHid ? p[kFW3 + o * kFHid + k] : 0.0f);
+    const int h = k < kFHid ? hidden_of_position(k) : 0;
+    w[kFPackW2 + i] = bf16_bits(k < kFHid ? p[kFW2 + o * kFHid + h] : 0.0f);
+    w[kFPackW3 + i] = bf16_bits(k < kFHid ? p[kFW3 + o * kFHid + h] : 0.0f);
   }
   if (i < kFOutPad * kFLdH) {  // W4 [16][136], rows 6..15 zero
     const int o = i / kFLdH, k = i - o * kFLdH;
-    w[kFPackW4 + i] = bf16_bits(o < kFOut && k < kFHid ? p[kFW4 + o * kFHid + k] : 0.0f);
+    const int h = k < kFHid ? hidden_of_position(k) : 0;
+    w[kFPackW4 + i] = bf16_bits(o < kFOut && k < kFHid ? p[kFW4 + o * kFHid + h] : 0.0f);
   }
   if (i < 3 * kFHid) bias[i] = p[(i < kFHid ? kFB1 : i < 2 * kFHid ? kFB2 - kFHid : kFB3 - 2 * kFHid) + i];
   if (i < kFOutPad) bias[3 * kFHid + i] = i < kFOut ? p[kFB4 + i] : 0.0f;
 }
 
-constexpr int kMlpWaves = 4;
+constexpr int kMlpWaves = 8;
 constexpr int kMlpThreads = 64 * kMlpWaves;
 constexpr int kRowsPerWave = 64;           // pairs (rows) a wave carries through the network
-constexpr int kTileLd = kFLdH;             // activation tile row stride (bf16)
+constexpr int kTileLd = kFLd1;             // feature tile row stride (bf16): 32 inputs + 8 pad
 constexpr int kTileElems = kRowsPerWave * kTileLd;
 constexpr size_t kMlpLds = (size_t)kFPackN * 2 + (size_t)kMlpWaves * kTileElems * 2 + (3 * kFHid + kFOutPad) * 4;
 
-// One wave runs the network on the 64 rows of its LDS tile h (row = pair, features in columns
-// 0..31, bf16). Each layer is computed transposed, D = W H^T, with the weights as the A operand
-// and the activations as the B operand of v_mfma_f32_16x16x32_bf16: lane l holds
-// A[l%16][8(l/16)..+7] (a weight row), B[8(l/16)..+7][l%16] (a pair's inputs) and
-// D[4(l/16)+i][l%16] (four consecutive outputs of one pair), so every operand read is 16 bytes
-// and every activation write-back 8 bytes. Returns out[pt][i] = output 4(l/16)+i of pair
-// 16 pt + l%16 (outputs 0..5 used). Every wave of the workgroup must call it together.
-__device__ __forceinline__ void mlp_tile64(const unsigned short* W, const float* bias, unsigned short* h, int lane,
-                                           f32x4 out[4]) {
+// Orders a wave's LDS writes before its following LDS reads (lanes read other lanes' rows).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wave runs the network on pairs 16 pt0 .. 16 (pt0 + NPT) - 1 of its LDS feature tile h
+// (row = pair, 13 features in columns 0..12 of 32, bf16). Each layer is computed transposed, D = W H^T, with the weights
+// (LDS) as the A operand and the activations as the B operand of v_mfma_f32_16x16x32_bf16:
+// lane l holds A[l%16][8(l/16)..+7] (a weight row), B[8(l/16)..+7][l%16] (one pair's inputs)
+// and D[4(l/16)+i][l%16] (four outputs of one pair). A layer's accumulators become the next
+// layer's B operand in the same lanes (SiLU, then bf16 pairs), with the next layer's weights
+// packed in that input order, so activations never leave the registers. Returns
+// out[p][i] = output 4(l/16)+i of pair 16 (pt0 + p) + l%16 (outputs 0..5 used).
+template <int NPT>
+__device__ __forceinline__ void mlp_tile(const unsigned short* W, const float* bias, const unsigned short* h, int pt0,
+                                         int lane, f32x4 out[NPT]) {
   const int r16 = lane & 15, kq = lane >> 4;
-  f32x4 acc[8][4];
+  f32x4 acc[8][NPT];
   // every layer's accumulators start at its bias (outputs 16 mt + 4 kq .. +3)
 #pragma unroll
   for (int mt = 0; mt < 8; mt++) {
     const float4 bv = *reinterpret_cast<const float4*>(bias + mt * 16 + 4 * kq);
 #pragma unroll
-    for (int pt = 0; pt < 4; pt++) acc[mt][pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
+    for (int pt = 0; pt < NPT; pt++) acc[mt][pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
   }
-  {  // layer 1: K = 32
-    bf16x8 b[4];
+  {  // layer 1: K = 32 from the feature tile
+    bf16x8 b[NPT];
 #pragma unroll
-    for (int pt = 0; pt < 4; pt++) b[pt] = *reinterpret_cast<const bf16x8*>(h + (pt * 16 + r16) * kTileLd + 8 * kq);
+    for (int pt = 0; pt < NPT; pt++) b[pt] = *reinterpret_cast<const bf16x8*>(h + ((pt0 + pt) * 16 + r16) * kTileLd + 8 * kq);
 #pragma unroll
     for (int mt = 0; mt < 8; mt++) {
       const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + kFPackW1 + (mt * 16 + r16) * kFLd1 + 8 * kq);
 #pragma unroll
-      for (int pt = 0; pt < 4; pt++) acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[pt], acc[mt][pt], 0, 0, 0);
+      for (int pt = 0; pt < NPT; pt++) acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[pt], acc[mt][pt], 0, 0, 0);
     }
   }
-  __syncthreads();  // every lane has read its layer input
 #pragma unroll
   for (int layer = 0; layer < 3; layer++) {
-    // SiLU, back to the tile as bf16: pair 16 pt + r16, outputs 16 mt + 4 kq .. +3
+    // SiLU, to bf16 in the registers: K-block ks of the next layer is tiles 2 ks and 2 ks + 1
+    bf16x8 b[4][NPT];
 #pragma unroll
-    for (int mt = 0; mt < 8; mt++)
+    for (int ks = 0; ks < 4; ks++)
 #pragma unroll
-      for (int pt = 0; pt < 4; pt++) {
-        const unsigned lo = pack_bf16(silu_fast(acc[mt][pt][0]), silu_fast(acc[mt][pt][1]));
-        const unsigned hi = pack_bf16(silu_fast(acc[mt][pt][2]), silu_fast(acc[mt][pt][3]));
-        *reinterpret_cast<uint2*>(h + (pt * 16 + r16) * kTileLd + mt * 16 + 4 * kq) = make_uint2(lo, hi);
+      for (int pt = 0; pt < NPT; pt++) {
+        const f32x4 u = acc[2 * ks][pt], v = acc[2 * ks + 1][pt];
+        const unsigned w0 = pack_bf16(silu_fast(u[0]), silu_fast(u[1]));
+        const unsigned w1 = pack_bf16(silu_fast(u[2]), silu_fast(u[3]));
+        const unsigned w2 = pack_bf16(silu_fast(v[0]), silu_fast(v[1]));
+        const unsigned w3 = pack_bf16(silu_fast(v[2]), silu_fast(v[3]));
+        b[ks][pt] = __builtin_bit_cast(bf16x8, make_uint4(w0, w1, w2, w3));
       }
-    __syncthreads();
     const int wofs = layer == 0 ? kFPackW2 : layer == 1 ? kFPackW3 : kFPackW4;
     const int mts = layer == 2 ? 1 : 8;
     const float* bnext = bias + (layer + 1) * kFHid;
@@ -365,26 +388,23 @@ __device__ __forceinline__ void mlp_tile64(const unsigned short* W, const float*
       if (mt >= mts) break;
       const float4 bv = *reinterpret_cast<const float4*>(bnext + mt * 16 + 4 * kq);
 #pragma unroll
-      for (int pt = 0; pt < 4; pt++) acc[mt][pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
+      for (int pt = 0; pt < NPT; pt++) acc[mt][pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
     }
 #pragma unroll
-    for (int ks = 0; ks < 4; ks++) {
-      bf16x8 b[4];
-#pragma unroll
-      for (int pt = 0; pt < 4; pt++)
-        b[pt] = *reinterpret_cast<const bf16x8*>(h + (pt * 16 + r16) * kTileLd + 32 * ks + 8 * kq);
+    for (int ks = 0; ks < 4; ks++)
 #pragma unroll
       for (int mt = 0; mt < 8; mt++) {
         if (mt >= mts) break;
         const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + wofs + (mt * 16 + r16) * kFLdH + 32 * ks + 8 * kq);
 #pragma unroll
-        for (int pt = 0; pt < 4; pt++) acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[pt], acc[mt][pt], 0, 0, 0);
+        for (int pt = 0; pt < NPT; pt++)
+          acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ks][pt], acc[mt][pt], 0, 0, 0);
+        // keep the weight loads of a K-block next to their products (register pressure)
+        if (mt == 3 || mt == 7) __builtin_amdgcn_sched_barrier(0);
       }
-    }
-    __syncthreads();
   }
 #pragma unroll
-  for (int pt = 0; pt < 4; pt++) out[pt] = acc[0][pt];
+  for (int pt = 0; pt < NPT; pt++) out[pt] = acc[0][pt];
 }
 
 // Loads the packed weights and biases into LDS.
@@ -428,7 +448,7 @@ __global__ __launch_bounds__(kMlpThreads) void k_mlp_rows(const unsigned short* 
   }
   __syncthreads();
   f32x4 out[4];
-  mlp_tile64(W, bias, h, lane, out);
+  mlp_tile<4>(W, bias, h, 0, lane, out);
   const int r16 = lane & 15, kq = lane >> 4;
 #pragma unroll
   for (int pt = 0; pt < 4; pt++) {
@@ -471,6 +491,7 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
   unsigned short* tiles = W + kFPackN;
   float* bias = reinterpret_cast<float*>(tiles + kMlpWaves * kTileElems);
   __shared__ double red[kMlpWaves][3 * kFMaxQ + 3];
+  __shared__ double fbs[kFMaxQ][3];
   __shared__ FarSpec spec;
   if (threadIdx.x == 0) spec = *f.spec;
   load_net(f.wpack, f.bias, W, bias);
@@ -479,17 +500,18 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
   const int r16 = lane & 15, kq = lane >> 4;
   unsigned short* h = tiles + wave * kTileElems;
   const int nq = d.topo.nq;
-  float qb[kFMaxQ][3];
-#pragma unroll
-  for (int a = 0; a < kFMaxQ; a++)
-    for (int c = 0; c < 3; c++) qb[a][c] = a < nq ? (float)spec.qbody[a][c] : 0.0f;
-  // this lane's outputs 4 kq + i: standardisation
-  float ymu[4], ysd[4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    ymu[i] = 4 * kq + i < kFOut ? spec.ymu[4 * kq + i] : 0.0f;
-    ysd[i] = 4 * kq + i < kFOut ? spec.ysd[4 * kq + i] : 0.0f;
+  // charged beads (FP32) and the output standardisation in LDS; registers go to the network
+  __shared__ float qb[kFMaxQ][3];
+  __shared__ float ymu[kFOutPad], ysd[kFOutPad];
+  if (threadIdx.x < kFMaxQ * 3) {
+    const int a = threadIdx.x / 3, c = threadIdx.x % 3;
+    qb[a][c] = a < nq ? (float)spec.qbody[a][c] : 0.0f;
   }
+  if (threadIdx.x < kFOutPad) {
+    ymu[threadIdx.x] = threadIdx.x < kFOut ? spec.ymu[threadIdx.x] : 0.0f;
+    ysd[threadIdx.x] = threadIdx.x < kFOut ? spec.ysd[threadIdx.x] : 0.0f;
+  }
+  __syncthreads();
   const float rf = (float)spec.rf, lbf = (float)spec.lb;
   for (int A = blockIdx.x; A < d.Ns; A += gridDim.x) {
     const RepParams& P = d.rp[d.sub_rep[A]];
@@ -567,33 +589,40 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
         for (int a = 0; a < kFMaxQ; a++) w[a] = 0.0f;
       }
       put_features(h + lane * kTileLd, x, sc > 0.0f);
-      __syncthreads();
-      f32x4 out[4];
-      mlp_tile64(W, bias, h, lane, out);
+      wave_lds_sync();
+      // two halves of 32 pairs, so the network's registers fit beside the spread sums
+#pragma unroll 1
+      for (int half = 0; half < 2; half++) {
+        f32x4 out[2];
+        mlp_tile<2>(W, bias, h, 2 * half, lane, out);
 #pragma unroll
-      for (int pt = 0; pt < 4; pt++) {
-        const int src = pt * 16 + r16;  // the lane that built this pair
-        const float scp = __shfl(sc, src, 64);
-        float wp[kFMaxQ];
+        for (int pp = 0; pp < 2; pp++) {
+          const int pt = 2 * half + pp;
+          const int src = pt * 16 + r16;  // the lane that built this pair
+          const float scp = __shfl(sc, src, 64);
+          float wp[kFMaxQ];
 #pragma unroll
-        for (int a = 0; a < kFMaxQ; a++) wp[a] = __shfl(w[a], src, 64);
-        if (scp == 0.0f) continue;
-        const float v0 = (ymu[0] + ysd[0] * out[pt][0]) * scp, v1 = (ymu[1] + ysd[1] * out[pt][1]) * scp,
-                    v2 = (ymu[2] + ysd[2] * out[pt][2]) * scp, v3 = (ymu[3] + ysd[3] * out[pt][3]) * scp;
-        if (kq == 0) {
+          for (int a = 0; a < kFMaxQ; a++) wp[a] = __shfl(w[a], src, 64);
+          if (scp == 0.0f) continue;
+          const float v0 = (ymu[4 * kq] + ysd[4 * kq] * out[pp][0]) * scp,
+                      v1 = (ymu[4 * kq + 1] + ysd[4 * kq + 1] * out[pp][1]) * scp,
+                      v2 = (ymu[4 * kq + 2] + ysd[4 * kq + 2] * out[pp][2]) * scp,
+                      v3 = (ymu[4 * kq + 3] + ysd[4 * kq + 3] * out[pp][3]) * scp;
+          if (kq == 0) {
 #pragma unroll
-          for (int a = 0; a < kFMaxQ; a++) {
-            accq[0][a] += (double)(v0 * wp[a]);
-            accq[1][a] += (double)(v1 * wp[a]);
-            accq[2][a] += (double)(v2 * wp[a]);
+            for (int a = 0; a < kFMaxQ; a++) {
+              accq[0][a] += (double)(v0 * wp[a]);
+              accq[1][a] += (double)(v1 * wp[a]);
+              accq[2][a] += (double)(v2 * wp[a]);
+            }
+            acct[0] += (double)v3;
+          } else if (kq == 1) {
+            acct[1] += (double)v0;
+            acct[2] += (double)v1;
           }
-          acct[0] += (double)v3;
-        } else if (kq == 1) {
-          acct[1] += (double)v0;
-          acct[2] += (double)v1;
         }
       }
-      __syncthreads();
+      wave_lds_sync();  // the tile's reads are done before the next round writes it
     }
     // fixed-order sums over the 16 lanes of a quarter-wave, then the waves
 #pragma unroll
@@ -606,8 +635,11 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
       for (int c = 0; c < 3; c++) acct[c] += __shfl_xor(acct[c], o, 64);
     }
     if (lane == 0) {
-      for (int a = 0; a < nq; a++)
-        for (int c = 0; c < 3; c++) red[wave][3 * a + c] = accq[c][a];
+#pragma unroll
+      for (int a = 0; a < kFMaxQ; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+          if (a < nq) red[wave][3 * a + c] = accq[c][a];
       red[wave][3 * kFMaxQ] = acct[0];
     } else if (lane == 16) {
       red[wave][3 * kFMaxQ + 1] = acct[1];
@@ -615,7 +647,8 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      double fb[kFMaxQ][3], T[3], Ts[3] = {0, 0, 0};
+      double (*fb)[3] = fbs;  // LDS: indexed by the runtime charge count
+      double T[3], Ts[3] = {0, 0, 0};
       for (int a = 0; a < nq; a++)
         for (int c = 0; c < 3; c++) {
           double s = 0.0;
