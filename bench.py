@@ -22,6 +22,8 @@ sys.path.insert(0, os.path.join(REPO, "tests"))
 
 HBM_PEAK_GBS = 8000.0    # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector
+# data/model trimer_MVM (charged), as shipped with the reference and copied under tests/golden
+TEMPLATE = os.path.join(REPO, "tests", "golden", "infiles", "trimer_MVM")
 CFG = dict(C_uM=800.0, c_mM=100.0, elj_att=2.1, dt=0.005, K=298.0, q_chain=5, Q=10.0, damp=100.0)
 
 
@@ -52,7 +54,6 @@ def measured_traffic(args):
 
 def make_system(args, brownian=False, fp32=False, skin=None):
     import souffle_gpu as sg
-    from oracle_lib import TEMPLATE
     t = sg.Template(TEMPLATE)
     p = t.params(n_subunits=args.subunits, brownian=1 if brownian else 0, **CFG)
     p.skin = args.skin if skin is None else skin
@@ -72,6 +73,11 @@ def extended_energy(s):
 def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
     s, pos, vel = make_system(args, brownian, fp32, skin)
     e_start = extended_energy(s)
+    tune = None
+    if args.tune_tile:
+        # the force kernel's pair tile by the library's autotuner at the fixed skin (trials run from
+        # a saved state that is restored; forces do not depend on the tile)
+        tune, _ = s.autotune(skins=[s.skin(0)], tiles=[128, 256, 512], trial_steps=200)
     s.step(args.warmup)
     st0 = s.stats()
     t0 = time.perf_counter()
@@ -85,6 +91,7 @@ def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
                n_list=(st1["lj_entries"] + st1["q_entries"]) / nb)
     out["k_rebuild"] = args.steps / max(out["rebuilds"], 1)
     out["e_start"], out["e_end"] = e_start, extended_energy(s)
+    out["tile"] = tune["tile"] if tune else None
     if timed_pass:
         s.set_timing(True)
         s.step(args.steps)
@@ -99,7 +106,7 @@ def run_cpu(args):
     centre prefilter, OpenMP dynamic over subunits, serial integrator) on the host cores, on a
     bounded sample of the same workload: `--cpu-steps` steps after one warm-up step."""
     import souffle_gpu as sg
-    from oracle_lib import TEMPLATE, Oracle
+    from oracle_lib import Oracle  # the CPU leg only
     t = sg.Template(TEMPLATE)
     p = t.params(n_subunits=args.subunits, **CFG)
     pos, vel = t.make_state(args.subunits, p.box, p.T, 2000)
@@ -143,6 +150,7 @@ def main():
     ap.add_argument("--cpu-replica-steps", type=int, default=400)
     ap.add_argument("--subunits", type=int, default=2000)
     ap.add_argument("--skin", type=float, default=0.3)
+    ap.add_argument("--tune-tile", type=int, default=1, help="autotune the force kernel's pair tile (0: library default)")
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -192,7 +200,6 @@ def bench_replicas(args):
     import numpy as np
     import souffle_gpu as sg
     import replica_sweep
-    from oracle_lib import TEMPLATE, Oracle
     t = sg.Template(TEMPLATE)
     sizes, params, pos, vel, draws = replica_sweep.sweep(t, args.replicas, 4)
     batch = sg.System(t, sizes, params)
@@ -244,6 +251,7 @@ def bench_replicas(args):
         "single_run_check": checked,
     }
     if not args.no_cpu:
+        from oracle_lib import Oracle  # the CPU leg only
         # the oracle on a seeded sample of the replicas, one after another on all host cores
         # (the sequential parameter-sweep equivalent), each at its smallest valid -L, -B 5
         rng = np.random.default_rng(9)
@@ -276,7 +284,6 @@ def run_surrogate(args, steps):
     seeded state; force error of the second against the first on one state."""
     import numpy as np
     import souffle_gpu as sg
-    from oracle_lib import TEMPLATE
     t = sg.Template(TEMPLATE)
     ns = args.sur_subunits
     model = sg.FarModel(t, 20.0, 5.0, 50.0, seed=2)
@@ -367,7 +374,7 @@ def bench_md(args):
         "data": "synthetic",
         "config": {"workload": f"trimer_MVM x{args.subunits} plain FP64 Nose-Hoover MD (q=5, Q=10), "
                                f"C=800uM c=100mM E=2.1 dt=0.005 K=298 ks=50 kb=20, skin {args.skin}",
-                   "subunits": args.subunits, "beads": md["nb"], "parallelism": "single GPU"},
+                   "subunits": args.subunits, "beads": md["nb"], "tile": md["tile"], "parallelism": "single GPU"},
         "roofline": roofline_entry(md["nb"], md["n_list"], md["k_rebuild"], t_force, md["value"],
                                    measured_traffic(args)),
         "kernels_ms": {"k_integrate": st["k_integrate_ms"], "k_rebuild": st["k_rebuild_ms"],

@@ -1198,6 +1198,58 @@ __device__ __forceinline__ PT yk_term(PT r2, PT qq, PT yk, PT kappa, PT ecut_el2
   return (qq * exp(-kappa * rr) / r2) * (kappa + (PT)1 / rr) * yk;
 }
 
+// What one pair-list entry of a force workgroup needs besides its neighbour's position: the
+// round's owner map and the per-bead offsets in LDS, the template tables, the subunit's own
+// positions, the Yukawa tables and the cutoffs (passed by value; everything is uniform).
+template <typename PT>
+struct PairCtx {
+  const unsigned char* owner;
+  const int* off;
+  const int* nlj;
+  const BeadLds* tb;
+  const double4* own;
+  const double* qq;
+  const double* yk;
+  const double4* centres;
+  double4 cs;
+  double box, hbox, far_rf2;
+  PT ecut2, elj, kappa, ecut_el2;
+  int nq, split;
+};
+
+// Pair term of entry c0 + le of the subunit's concatenated lists (LJ entries, then Yukawa
+// entries, bead by bead) on its bead from the neighbour at pj (src/forces.cpp:67-139); the
+// force components go to pf[le], pf[kChunk + le], pf[2 kChunk + le].
+template <typename PT, int kChunk>
+__device__ __forceinline__ void pair_entry(int le, int c0, double4 pj, PairCtx<PT> c, PT* pf) {
+  const int i = c.owner[le];
+  const int k = c0 + le - c.off[i];
+  const int nl = c.nlj[i];
+  const double4 p0 = c.own[i];
+  const PT dx = (PT)min_img(p0.x - pj.x, c.box, c.hbox), dy = (PT)min_img(p0.y - pj.y, c.box, c.hbox),
+           dz = (PT)min_img(p0.z - pj.z, c.box, c.hbox);
+  const PT r2 = dx * dx + dy * dy + dz * dz;
+  const int lj = tag_loc(pj.w);
+  PT f;
+  if (k < nl) {
+    const int tj = c.tb->type[lj];
+    f = lj_term<PT>(r2, (c.tb->mask[c.tb->type[i]] >> tj) & 1ull, tj, (PT)c.tb->sigma[i], (PT)c.tb->sigma[lj], c.ecut2,
+                    c.elj);
+  } else {
+    const int qs = c.tb->qslot[i], qj = c.tb->qslot[lj];
+    f = yk_term<PT>(r2, (PT)c.qq[qs * c.nq + qj], (PT)c.yk[qs * c.nq + qj], c.kappa, c.ecut_el2);
+    if (c.split) {  // uncut Yukawa: only subunit pairs within far_rf are near field
+      const double4 cb = c.centres[tag_sub(pj.w)];
+      const double cx = min_img(c.cs.x - cb.x, c.box, c.hbox), cy = min_img(c.cs.y - cb.y, c.box, c.hbox),
+                   cz = min_img(c.cs.z - cb.z, c.box, c.hbox);
+      if (cx * cx + cy * cy + cz * cz > c.far_rf2) f = (PT)0;
+    }
+  }
+  pf[le] = dx * f;
+  pf[kChunk + le] = dy * f;
+  pf[2 * kChunk + le] = dz * f;
+}
+
 // LDS bytes of k_force: own positions, the hinge forces (three beads per hinge; the fourth is
 // minus their sum) sharing their space with the pair terms of one gather round of `tile`
 // entries (the bending sums are taken before the pair rounds), and the Yukawa tables.
@@ -1207,7 +1259,8 @@ static inline size_t force_lds_bytes(int np, int nh, int nq, int tile) {
 
 // kChunk: list entries the workgroup evaluates per gather round (the tile).
 template <int MODE, typename PT, int kChunk>
-__global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
+// Four waves per SIMD is what the LDS allows (8 workgroups per CU): up to 128 VGPRs.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_force(Dev d, int init) {
   extern __shared__ double smem[];
   __shared__ double red[2][kPartStride];
   __shared__ BeadLds tb;
@@ -1275,6 +1328,28 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
   off_s[tid] = off_i;
   nlj_s[tid] = nlj;
   if (tid == 0) off_s[kBlock] = total;
+  // the first pair round's owner map, list entries and neighbour positions are requested now,
+  // so that their latency overlaps the bonded terms
+  constexpr int kPre = kChunk / kBlock < 2 ? kChunk / kBlock : 2;  // entry slots prefetched per thread
+  const int c1_first = min(total, kChunk);
+  if (active)
+    for (int e = off_i; e < min(off_i + n_i, c1_first); e++) owner_s[e] = (unsigned char)tid;
+  __syncthreads();
+  static_assert(kChunk % kBlock == 0 && kPre >= 1, "tile of a multiple of the workgroup size");
+  double4 pre0 = make_double4(0, 0, 0, 0), pre1 = pre0;  // no indexed array
+#pragma unroll
+  for (int u = 0; u < kPre; u++) {
+    const int e = tid + u * kBlock;
+    if (e < c1_first) {
+      const int i = owner_s[e];
+      const int k = e - off_s[i];
+      const int nl = nlj_s[i];
+      const int j = (k < nl) ? d.lj_list[(size_t)k * d.Nb + s * np + i]
+                             : d.q_list[(size_t)(k - nl) * d.Nq + (s * nq + tb.qslot[i])];
+      const double4 v = d.pos[j];
+      if (u == 0) pre0 = v; else pre1 = v;
+    }
+  }
   // bending hinges -> LDS
   for (int h = tid; h < nh; h += kBlock) {
     const int4 hg = (h == tid) ? hg0 : reinterpret_cast<const int4*>(d.topo.hinge)[h];
@@ -1324,43 +1399,32 @@ __global__ __launch_bounds__(kBlock) void k_force(Dev d, int init) {
   // then adds its own entries in list order, the sum of a sequential per-bead loop
   PT lx = 0, ly = 0, lz = 0, ex = 0, ey = 0, ez = 0;
   {
-    const PT ecut2 = (PT)P.ecut2, elj = (PT)P.elj_att, kappa = (PT)P.kappa, ecut_el2 = (PT)P.ecut_el2;
+    PairCtx<PT> ctx;
+    ctx.owner = owner_s; ctx.off = off_s; ctx.nlj = nlj_s; ctx.tb = &tb; ctx.own = own; ctx.qq = qq_s; ctx.yk = yk_s;
+    ctx.centres = d.centres; ctx.cs = cs; ctx.box = box; ctx.hbox = hbox; ctx.far_rf2 = P.far_rf2;
+    ctx.ecut2 = (PT)P.ecut2; ctx.elj = (PT)P.elj_att; ctx.kappa = (PT)P.kappa; ctx.ecut_el2 = (PT)P.ecut_el2;
+    ctx.nq = nq; ctx.split = P.split;
     for (int c0 = 0; c0 < total; c0 += kChunk) {
       __syncthreads();
       const int c1 = min(total, c0 + kChunk);
-      if (active)
+      if (active && c0 > 0)  // the first round's map was made before the bonded terms
         for (int e = max(off_i, c0); e < min(off_i + n_i, c1); e++) owner_s[e - c0] = (unsigned char)tid;
       __syncthreads();
-      for (int e = c0 + tid; e < c1; e += kBlock) {
-        const int i = owner_s[e - c0];
-        const int k = e - off_s[i];
-        const int nl = nlj_s[i];
-        const int bi = s * np + i;
-        const int qs = tb.qslot[i];
-        const int j = (k < nl) ? d.lj_list[(size_t)k * d.Nb + bi] : d.q_list[(size_t)(k - nl) * d.Nq + (s * nq + qs)];
-        const double4 pj = d.pos[j];
-        const double4 p0 = own[i];
-        const PT dx = (PT)min_img(p0.x - pj.x, box, hbox), dy = (PT)min_img(p0.y - pj.y, box, hbox),
-                 dz = (PT)min_img(p0.z - pj.z, box, hbox);
-        const PT r2 = dx * dx + dy * dy + dz * dz;
-        const int lj = tag_loc(pj.w);
-        PT f;
-        if (k < nl) {
-          const int tj = tb.type[lj];
-          f = lj_term<PT>(r2, (tb.mask[tb.type[i]] >> tj) & 1ull, tj, (PT)tb.sigma[i], (PT)tb.sigma[lj], ecut2, elj);
-        } else {
-          const int qj = tb.qslot[lj];
-          f = yk_term<PT>(r2, (PT)qq_s[qs * nq + qj], (PT)yk_s[qs * nq + qj], kappa, ecut_el2);
-          if (P.split) {  // uncut Yukawa: only subunit pairs within far_rf are near field
-            const double4 cb = d.centres[tag_sub(pj.w)];
-            const double cx = min_img(cs.x - cb.x, box, hbox), cy = min_img(cs.y - cb.y, box, hbox),
-                         cz = min_img(cs.z - cb.z, box, hbox);
-            if (cx * cx + cy * cy + cz * cz > P.far_rf2) f = (PT)0;
-          }
+      int e_next = c0 + tid;
+      if (c0 == 0) {  // the prefetched positions, one call per entry slot (no indexed array)
+        if (tid < c1) pair_entry<PT, kChunk>(tid, 0, pre0, ctx, pf);
+        if (kPre > 1 && tid + kBlock < c1) pair_entry<PT, kChunk>(tid + kBlock, 0, pre1, ctx, pf);
+        e_next = tid + kPre * kBlock;
+      }
+      {
+        for (int e = e_next; e < c1; e += kBlock) {
+          const int i = owner_s[e - c0];
+          const int k = e - off_s[i];
+          const int nl = nlj_s[i];
+          const int j = (k < nl) ? d.lj_list[(size_t)k * d.Nb + s * np + i]
+                                 : d.q_list[(size_t)(k - nl) * d.Nq + (s * nq + tb.qslot[i])];
+          pair_entry<PT, kChunk>(e - c0, c0, d.pos[j], ctx, pf);
         }
-        pf[e - c0] = dx * f;
-        pf[kChunk + e - c0] = dy * f;
-        pf[2 * kChunk + e - c0] = dz * f;
       }
       __syncthreads();
       if (active) {
