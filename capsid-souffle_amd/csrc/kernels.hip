@@ -576,34 +576,69 @@ __global__ void k_cells_clear(Dev d) {
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < d.ncell_q; c += gridDim.x * blockDim.x) d.cnt_q[c] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_cell_fill(Dev d, int force_rebuild) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= d.Nb) return;
-  const int s = b / d.topo.np, l = b - s * d.topo.np;
+constexpr int kFillSlots = 256;  // LDS hash of the distinct cells of one subunit's beads (> kBlock)
+
+// Records of the subunit's beads into one grid: beads that share a cell are counted in an LDS
+// hash (atomicCAS on the key, linear probing; the table is never full, so every probe ends),
+// then one global atomic per distinct cell reserves its slots and each bead writes at its rank.
+__device__ __forceinline__ void fill_grid(bool in, int c, const float4& rec, int* cnt, float4* cells, int cap, int* need,
+                                          int* key, int* cntl, int* base, int tid) {
+  for (int i = tid; i < kFillSlots; i += kBlock) { key[i] = -1; cntl[i] = 0; }
+  __syncthreads();
+  int h = 0, rank = 0;
+  if (in) {
+    h = (int)(((unsigned)c * 2654435761u) >> 24);
+    while (true) {
+      const int k = atomicCAS(&key[h], -1, c);
+      if (k == -1 || k == c) break;
+      h = (h + 1) & (kFillSlots - 1);
+    }
+    rank = atomicAdd(&cntl[h], 1);
+  }
+  __syncthreads();
+  for (int i = tid; i < kFillSlots; i += kBlock)
+    if (key[i] >= 0) {
+      base[i] = atomicAdd(&cnt[key[i]], cntl[i]);
+      if (base[i] + cntl[i] > cap) atomicMax(need, base[i] + cntl[i]);
+    }
+  __syncthreads();
+  if (in) {
+    const int slot = base[h] + rank;
+    if (slot < cap) cells[(size_t)c * cap + slot] = rec;
+  }
+  __syncthreads();  // the tables are reused by the next grid
+}
+
+// One workgroup per subunit of a rebuilding replica: its beads into the LJ grid and its charged
+// beads into the Yukawa grid, as FP32 records (x, y, z, id) of the wrapped positions.
+__global__ __launch_bounds__(kBlock) void k_cell_fill(Dev d, int force_rebuild) {
+  __shared__ int key[kFillSlots], cntl[kFillSlots], base[kFillSlots];
+  const int s = blockIdx.x, tid = threadIdx.x;
   const int r = d.sub_rep[s];
   RepState& S = d.rs[r];
   if (!force_rebuild && S.rebuild == 0) return;
   const RepParams& P = d.rp[r];
-  const double4 p = d.pos[b];
+  const int np = d.topo.np;
+  const bool active = tid < np;
+  const int b = s * np + tid;
   const float box = (float)P.box, hbox = (float)P.hbox;
-  const float x = wrapf((float)p.x, box, hbox), y = wrapf((float)p.y, box, hbox), z = wrapf((float)p.z, box, hbox);
-  const float4 rec = make_float4(x, y, z, __int_as_float(b));
-  {
-    const int g = P.g_lj;
-    const int c = P.c0_lj + (cell_of(z, hbox, P.inv_a_lj, g) * g + cell_of(y, hbox, P.inv_a_lj, g)) * g +
-                  cell_of(x, hbox, P.inv_a_lj, g);
-    const int slot = atomicAdd(&d.cnt_lj[c], 1);
-    if (slot < d.ccap_lj) d.cell_lj[(size_t)c * d.ccap_lj + slot] = rec;
-    else atomicMax(&S.need_cell_lj, slot + 1);
+  float4 rec = make_float4(0, 0, 0, 0);
+  int clj = 0, cq = 0;
+  bool charged = false;
+  if (active) {
+    const double4 p = d.pos[b];
+    const float x = wrapf((float)p.x, box, hbox), y = wrapf((float)p.y, box, hbox), z = wrapf((float)p.z, box, hbox);
+    rec = make_float4(x, y, z, __int_as_float(b));
+    int g = P.g_lj;
+    clj = P.c0_lj + (cell_of(z, hbox, P.inv_a_lj, g) * g + cell_of(y, hbox, P.inv_a_lj, g)) * g +
+          cell_of(x, hbox, P.inv_a_lj, g);
+    charged = d.topo.qslot[tid] >= 0;
+    g = P.g_q;
+    cq = P.c0_q + (cell_of(z, hbox, P.inv_a_q, g) * g + cell_of(y, hbox, P.inv_a_q, g)) * g + cell_of(x, hbox, P.inv_a_q, g);
   }
-  if (d.topo.qslot[l] >= 0) {
-    const int g = P.g_q;
-    const int c = P.c0_q + (cell_of(z, hbox, P.inv_a_q, g) * g + cell_of(y, hbox, P.inv_a_q, g)) * g +
-                  cell_of(x, hbox, P.inv_a_q, g);
-    const int slot = atomicAdd(&d.cnt_q[c], 1);
-    if (slot < d.ccap_q) d.cell_q[(size_t)c * d.ccap_q + slot] = rec;
-    else atomicMax(&S.need_cell_q, slot + 1);
-  }
+  fill_grid(active, clj, rec, d.cnt_lj, d.cell_lj, d.ccap_lj, &S.need_cell_lj, key, cntl, base, tid);
+  if (d.topo.nq > 0)
+    fill_grid(active && charged, cq, rec, d.cnt_q, d.cell_q, d.ccap_q, &S.need_cell_q, key, cntl, base, tid);
 }
 
 constexpr int kCqRecords = 512;   // LDS records (other subunits) of the cells around one subunit
@@ -1839,7 +1874,7 @@ int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st) {
     hipLaunchKernelGGL(k_cells_clear, dim3(256), dim3(256), 0, st, d);
     if (int rc = check_launch()) return rc;
   }
-  hipLaunchKernelGGL(k_cell_fill, dim3((d.Nb + 255) / 256), dim3(256), 0, st, d, force_rebuild);
+  hipLaunchKernelGGL(k_cell_fill, dim3(d.Ns), dim3(kBlock), 0, st, d, force_rebuild);
   if (int rc = check_launch()) return rc;
   hipLaunchKernelGGL(k_cell_query, dim3(d.Ns), dim3(kBlock), 0, st, d, force_rebuild);
   return check_launch();
