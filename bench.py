@@ -93,8 +93,9 @@ def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
     out["e_start"], out["e_end"] = e_start, extended_energy(s)
     out["tile"] = tune["tile"] if tune else None
     if timed_pass:
+        # per-kernel means: HIP events around every 10th step's kernels (direct launches)
         s.set_timing(True)
-        s.step(args.steps)
+        s.step(min(args.steps, args.event_steps))
         s.set_timing(False)
         out["stats"] = s.stats()
     out["system"] = s
@@ -150,6 +151,7 @@ def main():
     ap.add_argument("--cpu-replica-steps", type=int, default=400)
     ap.add_argument("--subunits", type=int, default=2000)
     ap.add_argument("--skin", type=float, default=0.3)
+    ap.add_argument("--event-steps", type=int, default=2000, help="steps of the event-timed pass (per-kernel means)")
     ap.add_argument("--tune-tile", type=int, default=1, help="autotune the force kernel's pair tile (0: library default)")
     ap.add_argument("--cpu-steps", type=int, default=150)
     ap.add_argument("--cpu-threads", type=int, default=0)
