@@ -1,15 +1,17 @@
-// HIP kernels for gfx950 (MI355X). One SOUFFLE step is four launches:
-//   k_integrate  one thread per bead: pending centre-of-mass removal, first half kick (NH) or
-//                drag + half kick (BD), drift, periodic wrap, list-displacement check
-//   k_rebuild    one 128-thread workgroup per subunit, returns at once unless a bead of its
-//                replica moved more than skin/2: Verlet lists of the subunit's beads from a
-//                local LDS cell grid of the nearby subunits' beads
-//   k_force      one 128-thread workgroup per subunit: stretch + bending (hinge terms staged in
-//                LDS, gathered per bead through padded incidence rows) + LJ/WCA and Yukawa
-//                gathers over the lists + second half kick (NH) or noise + drag + half kick (BD)
-//                + per-subunit sums
-//   k_finalize   one workgroup per replica: fixed-order reductions, Nose-Hoover chain,
-//                centre-of-mass velocity, BD engine advance
+// HIP kernels for gfx950 (MI355X). One SOUFFLE step is five launches:
+//   k_integrate   one thread per bead: pending centre-of-mass removal, first half kick (NH) or
+//                 drag + half kick (BD), drift, periodic wrap, list-displacement check, cell counts
+//                 cleared for the next build
+//   k_cell_fill   one 128-thread workgroup per subunit, returns at once unless a bead of its
+//   k_cell_query  replica moved more than skin/2: cell lists of FP32 records, then each subunit's
+//                 Verlet lists from the records of other subunits gathered into LDS (cut Yukawa;
+//                 the uncut Yukawa's near lists come from k_rebuild)
+//   k_force       one 128-thread workgroup per subunit: stretch + bending (hinge terms staged in
+//                 LDS, gathered per bead through padded incidence rows) + LJ/WCA and Yukawa
+//                 gathers over the lists + second half kick (NH) or noise + drag + half kick (BD)
+//                 + per-subunit sums
+//   k_finalize    one workgroup per replica: fixed-order reductions, Nose-Hoover chain,
+//                 centre-of-mass velocity, BD engine advance
 // No atomics touch forces or sums: each bead's force is gathered by one thread, its lists are
 // in ascending bead id, and every reduction has a fixed shape, so runs are bit-reproducible and
 // do not depend on when lists were rebuilt.
