@@ -1174,7 +1174,7 @@ __device__ __forceinline__ double hinge_forces(const double4* own, int4 hg, doub
                bz = min_img(p3.z - p0.z, box, hbox);
   const double ux = ey * az - ez * ay, uy = ez * ax - ex * az, uz = ex * ay - ey * ax;
   const double vx = ey * bz - ez * by, vy = ez * bx - ex * bz, vz = ex * by - ey * bx;
-  const double iu = 1.0 / sqrt(ux * ux + uy * uy + uz * uz), iv = 1.0 / sqrt(vx * vx + vy * vy + vz * vz);
+  const double iu = rsqrt(ux * ux + uy * uy + uz * uz), iv = rsqrt(vx * vx + vy * vy + vz * vz);
   const double c = (ux * vx + uy * vy + uz * vz) * iu * iv;
   const double gx = (vx * iv - ux * (c * iu)) * iu, gy = (vy * iv - uy * (c * iu)) * iu,
                gz = (vz * iv - uz * (c * iu)) * iu;
@@ -1408,8 +1408,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
       const double4 po = own[o];
       const double dx = min_img(pi.x - po.x, box, hbox), dy = min_img(pi.y - po.y, box, hbox),
                    dz = min_img(pi.z - po.z, box, hbox);
-      const double L = sqrt(dx * dx + dy * dy + dz * dz);
-      const double c = (l0 - L) * ks / L;
+      const double L2 = dx * dx + dy * dy + dz * dz, iL = rsqrt(L2), L = L2 * iL;
+      const double c = (l0 - L) * ks * iL;
       sx += dx * c; sy += dy * c; sz += dz * c;
     }
   }
