@@ -274,8 +274,13 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// SiLU for inference: z / (1 + e^-z) with the hardware exp and reciprocal.
-__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+// SiLU for inference on log2(e)-scaled pre-activations: u = log2(e) z gives
+// u / (1 + 2^-u) = log2(e) silu(z) with the bare hardware exp2 and reciprocal. The packed
+// weights carry the factors (layer-1 rows and hidden biases times log2(e), output rows times
+// ln 2; the hidden-to-hidden rows need none, since ln 2 log2(e) = 1).
+__device__ __forceinline__ float silu_l2(float u) {
+  return u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-u));
+}
 
 // Two floats to packed bf16 (round to nearest even, v_cvt_pk_bf16_f32).
 __device__ __forceinline__ unsigned pack_bf16(float a, float b) {
@@ -301,9 +306,10 @@ __device__ __forceinline__ int hidden_of_position(int k) {
 
 __global__ void k_pack_weights(const float* p, unsigned short* w, float* bias) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const float log2e = 1.4426950408889634f, ln2 = 0.6931471805599453f;  // exp2-based SiLU (silu_l2)
   if (i < kFHid * kFLd1) {  // W1 [128][40], inputs 13..39 zero
     const int o = i / kFLd1, k = i - o * kFLd1;
-    w[kFPackW1 + i] = bf16_bits(k < kFIn ? p[kFW1 + o * kFIn + k] : 0.0f);
+    w[kFPackW1 + i] = bf16_bits(k < kFIn ? log2e * p[kFW1 + o * kFIn + k] : 0.0f);
   }
   if (i < kFHid * kFLdH) {  // W2, W3 [128][136], inputs in the accumulator order
     const int o = i / kFLdH, k = i - o * kFLdH;
@@ -314,9 +320,9 @@ __global__ void k_pack_weights(const float* p, unsigned short* w, float* bias) {
   if (i < kFOutPad * kFLdH) {  // W4 [16][136], rows 6..15 zero
     const int o = i / kFLdH, k = i - o * kFLdH;
     const int h = k < kFHid ? hidden_of_position(k) : 0;
-    w[kFPackW4 + i] = bf16_bits(o < kFOut && k < kFHid ? p[kFW4 + o * kFHid + h] : 0.0f);
+    w[kFPackW4 + i] = bf16_bits(o < kFOut && k < kFHid ? ln2 * p[kFW4 + o * kFHid + h] : 0.0f);
   }
-  if (i < 3 * kFHid) bias[i] = p[(i < kFHid ? kFB1 : i < 2 * kFHid ? kFB2 - kFHid : kFB3 - 2 * kFHid) + i];
+  if (i < 3 * kFHid) bias[i] = log2e * p[(i < kFHid ? kFB1 : i < 2 * kFHid ? kFB2 - kFHid : kFB3 - 2 * kFHid) + i];
   if (i < kFOutPad) bias[3 * kFHid + i] = i < kFOut ? p[kFB4 + i] : 0.0f;
 }
 
@@ -335,76 +341,95 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // One wave runs the network on pairs 16 pt0 .. 16 (pt0 + NPT) - 1 of its LDS feature tile h
-// (row = pair, 13 features in columns 0..12 of 32, bf16). Each layer is computed transposed, D = W H^T, with the weights
-// (LDS) as the A operand and the activations as the B operand of v_mfma_f32_16x16x32_bf16:
-// lane l holds A[l%16][8(l/16)..+7] (a weight row), B[8(l/16)..+7][l%16] (one pair's inputs)
-// and D[4(l/16)+i][l%16] (four outputs of one pair). A layer's accumulators become the next
-// layer's B operand in the same lanes (SiLU, then bf16 pairs), with the next layer's weights
-// packed in that input order, so activations never leave the registers. Returns
+// (row = pair, 13 features in columns 0..12 of 32, bf16). Each layer is computed transposed,
+// D = W H^T, with the weights (LDS) as the A operand and the activations as the B operand of
+// v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16)..+7] (a weight row), B[8(l/16)..+7][l%16]
+// (one pair's inputs) and D[4(l/16)+i][l%16] (four outputs of one pair). Returns
 // out[p][i] = output 4(l/16)+i of pair 16 (pt0 + p) + l%16 (outputs 0..5 used).
+//
+// A hidden layer runs block by block: the 16 outputs of row block mt are complete after its
+// K-blocks, go through SiLU while the MFMAs of block mt + 2 run, and become half of bf16 K-block
+// mt/2 of the next layer's B operand in the same lanes (block 2k gives inputs 8g..8g+3 of lane
+// group g, block 2k+1 inputs 8g+4..8g+7; the next layer's weights are packed in that input
+// order), so activations never leave the registers and the vector ALU overlaps the matrix core.
+template <int NPT>
+__device__ __forceinline__ void silu_block(const f32x4 (&u)[NPT], unsigned (&w)[NPT][2]) {
+#pragma unroll
+  for (int pt = 0; pt < NPT; pt++) {
+    w[pt][0] = pack_bf16(silu_l2(u[pt][0]), silu_l2(u[pt][1]));
+    w[pt][1] = pack_bf16(silu_l2(u[pt][2]), silu_l2(u[pt][3]));
+  }
+}
+
+// One hidden layer (128 outputs) from KB input K-blocks into bout[4] (bf16 SiLU outputs).
+// Iteration mt requests block mt+1's weight fragments, issues block mt's MFMAs and the SiLU of
+// block mt-2 (independent, so the compiler interleaves them); a scheduling fence between
+// iterations bounds the registers in flight.
+template <int NPT, int KB>
+__device__ __forceinline__ void hidden_layer(const unsigned short* Wl, int ldw, const float* bl,
+                                             const bf16x8 (&bin)[KB][NPT], bf16x8 (&bout)[4][NPT], int r16, int kq) {
+  const unsigned short* wrow = Wl + r16 * ldw + 8 * kq;
+  bf16x8 a[KB], an[KB];
+#pragma unroll
+  for (int ks = 0; ks < KB; ks++) a[ks] = *reinterpret_cast<const bf16x8*>(wrow + 32 * ks);
+  f32x4 pend[2][NPT];
+  unsigned lo[NPT][2], hi[NPT][2];
+#pragma unroll
+  for (int mt = 0; mt < 10; mt++) {
+    if (mt < 7) {
+#pragma unroll
+      for (int ks = 0; ks < KB; ks++) an[ks] = *reinterpret_cast<const bf16x8*>(wrow + (mt + 1) * 16 * ldw + 32 * ks);
+    }
+    f32x4 acc[NPT];
+    if (mt < 8) {
+      const float4 bv = *reinterpret_cast<const float4*>(bl + mt * 16 + 4 * kq);
+#pragma unroll
+      for (int pt = 0; pt < NPT; pt++) acc[pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+      for (int ks = 0; ks < KB; ks++)
+#pragma unroll
+        for (int pt = 0; pt < NPT; pt++) acc[pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[ks], bin[ks][pt], acc[pt], 0, 0, 0);
+    }
+    if (mt >= 2) {  // SiLU of block mt - 2
+      if (mt & 1) {
+        silu_block<NPT>(pend[mt & 1], hi);
+#pragma unroll
+        for (int pt = 0; pt < NPT; pt++)
+          bout[(mt - 2) >> 1][pt] = __builtin_bit_cast(bf16x8, make_uint4(lo[pt][0], lo[pt][1], hi[pt][0], hi[pt][1]));
+      } else {
+        silu_block<NPT>(pend[mt & 1], lo);
+      }
+    }
+    if (mt < 8) {
+#pragma unroll
+      for (int pt = 0; pt < NPT; pt++) pend[mt & 1][pt] = acc[pt];
+#pragma unroll
+      for (int ks = 0; ks < KB; ks++) a[ks] = an[ks];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <int NPT>
 __device__ __forceinline__ void mlp_tile(const unsigned short* W, const float* bias, const unsigned short* h, int pt0,
                                          int lane, f32x4 out[NPT]) {
   const int r16 = lane & 15, kq = lane >> 4;
-  f32x4 acc[8][NPT];
-  // every layer's accumulators start at its bias (outputs 16 mt + 4 kq .. +3)
+  bf16x8 x[1][NPT], ba[4][NPT], bb[4][NPT];
 #pragma unroll
-  for (int mt = 0; mt < 8; mt++) {
-    const float4 bv = *reinterpret_cast<const float4*>(bias + mt * 16 + 4 * kq);
+  for (int pt = 0; pt < NPT; pt++) x[0][pt] = *reinterpret_cast<const bf16x8*>(h + ((pt0 + pt) * 16 + r16) * kTileLd + 8 * kq);
+  hidden_layer<NPT, 1>(W + kFPackW1, kFLd1, bias, x, ba, r16, kq);
+  hidden_layer<NPT, 4>(W + kFPackW2, kFLdH, bias + kFHid, ba, bb, r16, kq);
+  hidden_layer<NPT, 4>(W + kFPackW3, kFLdH, bias + 2 * kFHid, bb, ba, r16, kq);
+  // output layer: one 16-row block (rows 6..15 zero)
+  const float4 bv = *reinterpret_cast<const float4*>(bias + 3 * kFHid + 4 * kq);
 #pragma unroll
-    for (int pt = 0; pt < NPT; pt++) acc[mt][pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
+  for (int pt = 0; pt < NPT; pt++) out[pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) {
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + kFPackW4 + r16 * kFLdH + 32 * ks + 8 * kq);
+#pragma unroll
+    for (int pt = 0; pt < NPT; pt++) out[pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, ba[ks][pt], out[pt], 0, 0, 0);
   }
-  {  // layer 1: K = 32 from the feature tile
-    bf16x8 b[NPT];
-#pragma unroll
-    for (int pt = 0; pt < NPT; pt++) b[pt] = *reinterpret_cast<const bf16x8*>(h + ((pt0 + pt) * 16 + r16) * kTileLd + 8 * kq);
-#pragma unroll
-    for (int mt = 0; mt < 8; mt++) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + kFPackW1 + (mt * 16 + r16) * kFLd1 + 8 * kq);
-#pragma unroll
-      for (int pt = 0; pt < NPT; pt++) acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[pt], acc[mt][pt], 0, 0, 0);
-    }
-  }
-#pragma unroll
-  for (int layer = 0; layer < 3; layer++) {
-    // SiLU, to bf16 in the registers: K-block ks of the next layer is tiles 2 ks and 2 ks + 1
-    bf16x8 b[4][NPT];
-#pragma unroll
-    for (int ks = 0; ks < 4; ks++)
-#pragma unroll
-      for (int pt = 0; pt < NPT; pt++) {
-        const f32x4 u = acc[2 * ks][pt], v = acc[2 * ks + 1][pt];
-        const unsigned w0 = pack_bf16(silu_fast(u[0]), silu_fast(u[1]));
-        const unsigned w1 = pack_bf16(silu_fast(u[2]), silu_fast(u[3]));
-        const unsigned w2 = pack_bf16(silu_fast(v[0]), silu_fast(v[1]));
-        const unsigned w3 = pack_bf16(silu_fast(v[2]), silu_fast(v[3]));
-        b[ks][pt] = __builtin_bit_cast(bf16x8, make_uint4(w0, w1, w2, w3));
-      }
-    const int wofs = layer == 0 ? kFPackW2 : layer == 1 ? kFPackW3 : kFPackW4;
-    const int mts = layer == 2 ? 1 : 8;
-    const float* bnext = bias + (layer + 1) * kFHid;
-#pragma unroll
-    for (int mt = 0; mt < 8; mt++) {
-      if (mt >= mts) break;
-      const float4 bv = *reinterpret_cast<const float4*>(bnext + mt * 16 + 4 * kq);
-#pragma unroll
-      for (int pt = 0; pt < NPT; pt++) acc[mt][pt] = f32x4{bv.x, bv.y, bv.z, bv.w};
-    }
-#pragma unroll
-    for (int ks = 0; ks < 4; ks++)
-#pragma unroll
-      for (int mt = 0; mt < 8; mt++) {
-        if (mt >= mts) break;
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + wofs + (mt * 16 + r16) * kFLdH + 32 * ks + 8 * kq);
-#pragma unroll
-        for (int pt = 0; pt < NPT; pt++)
-          acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[ks][pt], acc[mt][pt], 0, 0, 0);
-        // keep the weight loads of a K-block next to their products (register pressure)
-        if (mt == 3 || mt == 7) __builtin_amdgcn_sched_barrier(0);
-      }
-  }
-#pragma unroll
-  for (int pt = 0; pt < NPT; pt++) out[pt] = acc[0][pt];
 }
 
 // Loads the packed weights and biases into LDS.
@@ -521,12 +546,13 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
     float RA[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) RA[k] = f.frames[(size_t)A * 12 + k];
-    // kq == 0 lanes: spread force (3 components x charged beads) and torque x; kq == 1: torque y, z
-    double accq[3][kFMaxQ], acct[3] = {0, 0, 0};
+    // kq == 0 lanes: spread force (3 components x charged beads) and torque x; kq == 1: torque y, z.
+    // FP32 per lane (a fixed order over the lane's pairs), FP64 across lanes and waves.
+    float accq[3][kFMaxQ], acct[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
-      for (int a = 0; a < kFMaxQ; a++) accq[c][a] = 0.0;
+      for (int a = 0; a < kFMaxQ; a++) accq[c][a] = 0.0f;
     for (int t0 = 0; t0 < P.ns; t0 += kRowsPerWave * kMlpWaves) {
       // this lane's pair: features, scale and spread weights
       const int t = t0 + wave * kRowsPerWave + lane;
@@ -542,29 +568,30 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
           const float fx = (float)cx, fy = (float)cy, fz = (float)cz;
           const float dv[3] = {-(RA[0] * fx + RA[3] * fy + RA[6] * fz), -(RA[1] * fx + RA[4] * fy + RA[7] * fz),
                                -(RA[2] * fx + RA[5] * fy + RA[8] * fz)};
-          float RB[9], Q[9];
-#pragma unroll
-          for (int k = 0; k < 9; k++) RB[k] = f.frames[(size_t)B * 12 + k];
+          const float4* rb4 = reinterpret_cast<const float4*>(f.frames + (size_t)B * 12);  // 48-byte rows
+          const float4 rb0 = rb4[0], rb1 = rb4[1], rb2 = rb4[2];
+          const float RB[9] = {rb0.x, rb0.y, rb0.z, rb0.w, rb1.x, rb1.y, rb1.z, rb1.w, rb2.x};
+          float Q[9];
 #pragma unroll
           for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++) Q[3 * i + j] = RA[i] * RB[j] + RA[3 + i] * RB[3 + j] + RA[6 + i] * RB[6 + j];
-          const float dd = sqrtf((float)d2), inv = 1.0f / dd;
+          const float inv = __builtin_amdgcn_rsqf((float)d2), dd = (float)d2 * inv;
           const float n[3] = {dv[0] * inv, dv[1] * inv, dv[2] * inv};
           const float wr = rf * inv;
           x[0] = n[0]; x[1] = n[1]; x[2] = n[2];
           x[3] = wr; x[4] = wr * wr; x[5] = wr * wr * wr;
           x[6] = Q[0]; x[7] = Q[3]; x[8] = Q[6]; x[9] = Q[1]; x[10] = Q[4]; x[11] = Q[7];
           x[12] = kappa;
-          // pair scale (far_scale) and spread weights
+          // pair scale (far_scale) and spread weights; B's charges project on n as -(Q qb).n = -qb.(Q^T n)
+          const float m[3] = {Q[0] * n[0] + Q[3] * n[1] + Q[6] * n[2], Q[1] * n[0] + Q[4] * n[1] + Q[7] * n[2],
+                              Q[2] * n[0] + Q[5] * n[1] + Q[8] * n[2]};
           float pa[kFMaxQ], ma = -1e30f, mb = -1e30f;
 #pragma unroll
           for (int a = 0; a < kFMaxQ; a++) {
             if (a >= nq) break;
             pa[a] = qb[a][0] * n[0] + qb[a][1] * n[1] + qb[a][2] * n[2];
-            const float pb = -((Q[0] * qb[a][0] + Q[1] * qb[a][1] + Q[2] * qb[a][2]) * n[0] +
-                               (Q[3] * qb[a][0] + Q[4] * qb[a][1] + Q[5] * qb[a][2]) * n[1] +
-                               (Q[6] * qb[a][0] + Q[7] * qb[a][1] + Q[8] * qb[a][2]) * n[2]);
+            const float pb = -(qb[a][0] * m[0] + qb[a][1] * m[1] + qb[a][2] * m[2]);
             w[a] = pb;  // reuse as scratch for p'_b
             ma = fmaxf(ma, pa[a]);
             mb = fmaxf(mb, pb);
@@ -577,7 +604,7 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
             w[a] = __expf(kappa * (pa[a] - ma));
             sa += w[a];
           }
-          const float isa = 1.0f / sa;
+          const float isa = __builtin_amdgcn_rcpf(sa);
 #pragma unroll
           for (int a = 0; a < kFMaxQ; a++) w[a] = a < nq ? w[a] * isa : 0.0f;
           sc = lbf * (kappa + inv) * inv * __expf(-kappa * (dd - ma - mb)) * sa * sb;
@@ -611,39 +638,46 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
           if (kq == 0) {
 #pragma unroll
             for (int a = 0; a < kFMaxQ; a++) {
-              accq[0][a] += (double)(v0 * wp[a]);
-              accq[1][a] += (double)(v1 * wp[a]);
-              accq[2][a] += (double)(v2 * wp[a]);
+              accq[0][a] = fmaf(v0, wp[a], accq[0][a]);
+              accq[1][a] = fmaf(v1, wp[a], accq[1][a]);
+              accq[2][a] = fmaf(v2, wp[a], accq[2][a]);
             }
-            acct[0] += (double)v3;
+            acct[0] += v3;
           } else if (kq == 1) {
-            acct[1] += (double)v0;
-            acct[2] += (double)v1;
+            acct[1] += v0;
+            acct[2] += v1;
           }
         }
       }
       wave_lds_sync();  // the tile's reads are done before the next round writes it
     }
-    // fixed-order sums over the 16 lanes of a quarter-wave, then the waves
+    // fixed-order FP64 sums over the 16 lanes of a quarter-wave, then the waves
+    double dq[3][kFMaxQ], dt3[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+      for (int a = 0; a < kFMaxQ; a++) dq[c][a] = (double)accq[c][a];
+      dt3[c] = (double)acct[c];
+    }
 #pragma unroll
     for (int o = 8; o > 0; o >>= 1) {
 #pragma unroll
       for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int a = 0; a < kFMaxQ; a++) accq[c][a] += __shfl_xor(accq[c][a], o, 64);
+        for (int a = 0; a < kFMaxQ; a++) dq[c][a] += __shfl_xor(dq[c][a], o, 64);
 #pragma unroll
-      for (int c = 0; c < 3; c++) acct[c] += __shfl_xor(acct[c], o, 64);
+      for (int c = 0; c < 3; c++) dt3[c] += __shfl_xor(dt3[c], o, 64);
     }
     if (lane == 0) {
 #pragma unroll
       for (int a = 0; a < kFMaxQ; a++)
 #pragma unroll
         for (int c = 0; c < 3; c++)
-          if (a < nq) red[wave][3 * a + c] = accq[c][a];
-      red[wave][3 * kFMaxQ] = acct[0];
+          if (a < nq) red[wave][3 * a + c] = dq[c][a];
+      red[wave][3 * kFMaxQ] = dt3[0];
     } else if (lane == 16) {
-      red[wave][3 * kFMaxQ + 1] = acct[1];
-      red[wave][3 * kFMaxQ + 2] = acct[2];
+      red[wave][3 * kFMaxQ + 1] = dt3[1];
+      red[wave][3 * kFMaxQ + 2] = dt3[2];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
