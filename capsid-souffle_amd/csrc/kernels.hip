@@ -1552,26 +1552,35 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
   double xi_l = 0.0;
   if (tid < kMaxChain && !init && !brownian) xi_l = S.xi_n[tid];
   // slot-major partials: a wave reads 64 consecutive subunits of one slot per load; each thread
-  // adds its subunits tid, tid + kFinThreads, ... in that order. Slots 4-6 (BD noise) and 7-8
-  // (list sizes, written at list builds) are read only when they are used.
+  // adds its subunits tid, tid + kFinThreads, ... in that order, with the loads of kFinBatch of
+  // them in flight together. Slots 4-6 (BD noise) and 7-8 (list sizes, written at list builds)
+  // are read and reduced only when they are used.
+  constexpr int kFinBatch = 4;
   double acc[kPartSlots];
 #pragma unroll
   for (int k = 0; k < kPartSlots; k++) acc[k] = 0.0;
   const int s_end = P.s0 + P.ns;
   const size_t ns_all = d.Ns;
-  for (int s = P.s0 + tid; s < s_end; s += kFinThreads) {
-    double v[kPartSlots];
+  for (int s0 = P.s0 + tid; s0 < s_end; s0 += kFinThreads * kFinBatch) {
+    double v[kFinBatch][kPartSlots];
 #pragma unroll
-    for (int k = 0; k < kPartSlots; k++) {
-      const bool used = k < 4 || (k < 7 ? brownian != 0 : lists);
-      v[k] = used ? d.part[k * ns_all + s] : 0.0;
+    for (int i = 0; i < kFinBatch; i++) {
+      const int s = s0 + i * kFinThreads;
+#pragma unroll
+      for (int k = 0; k < kPartSlots; k++) {
+        const bool used = s < s_end && (k < 4 || (k < 7 ? brownian != 0 : lists));
+        v[i][k] = used ? d.part[k * ns_all + s] : 0.0;
+      }
     }
 #pragma unroll
-    for (int k = 0; k < kPartSlots; k++) acc[k] += v[k];
+    for (int i = 0; i < kFinBatch; i++)
+#pragma unroll
+      for (int k = 0; k < kPartSlots; k++) acc[k] += v[i][k];
   }
 #pragma unroll
   for (int k = 0; k < kPartSlots; k++) {
-    const double w = wave_sum(acc[k]);
+    const bool used = k < 4 || (k < 7 ? brownian != 0 : lists);  // uniform over the workgroup
+    const double w = used ? wave_sum(acc[k]) : 0.0;
     if ((tid & 63) == 0) wred[tid >> 6][k] = w;
   }
   __syncthreads();
