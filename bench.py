@@ -74,11 +74,12 @@ def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
     s, pos, vel = make_system(args, brownian, fp32, skin)
     e_start = extended_energy(s)
     tune = None
-    if args.tune_tile:
-        # the force kernel's pair tile by the library's autotuner at the fixed skin (trials run from
-        # a saved state that is restored; forces do not depend on the tile)
-        tune, _ = s.autotune(skins=[s.skin(0)], tiles=[128, 256, 512], trial_steps=200)
     s.step(args.warmup)
+    if args.tune_tile:
+        # the force kernel's pair tile by the library's autotuner at the fixed skin, after the
+        # warm-up so that the trials see clusters forming (trials run from a saved state that is
+        # restored; forces do not depend on the tile)
+        tune, _ = s.autotune(skins=[s.skin(0)], tiles=[128, 256, 512], trial_steps=200)
     st0 = s.stats()
     t0 = time.perf_counter()
     s.step(args.steps)
@@ -144,7 +145,7 @@ def main():
     ap.add_argument("--sur-samples", type=int, default=1000000)
     ap.add_argument("--sur-epochs", type=int, default=20)
     ap.add_argument("--sur-steps", type=int, default=500, help="timed steps of the surrogate summary in the md line")
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (md 1000, others 200)")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--tune-steps", type=int, default=100)
     ap.add_argument("--cpu-replicas", type=int, default=8)
@@ -161,6 +162,8 @@ def main():
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"replicas": 3000, "surrogate": 1000}.get(args.workload, 20000)
+    if args.warmup is None:
+        args.warmup = 1000 if args.workload == "md" else 200
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one large system would shard "
               "into subunit-owning spatial domains with one halo exchange per step).", file=sys.stderr)
@@ -279,7 +282,7 @@ def bench_replicas(args):
     return out
 
 
-def run_surrogate(args, steps):
+def run_surrogate(args, steps, warmup):
     """Uncut Yukawa at 5000 subunits (C = 800 uM, c = args.sur_salt mM, other parameters as the
     default workload): the plain FP64 path (exact far field, subunit-pair tiles) against FP32 pair
     arithmetic with the far field from the network (trained here on the GPU), from the same
@@ -305,7 +308,7 @@ def run_surrogate(args, steps):
         s.set_state(pos, vel)
         if use_model:
             s.set_far_model(model, tol=1e-3)
-        s.step(100)
+        s.step(warmup)
         t1 = time.perf_counter()
         s.step(steps)
         wall = time.perf_counter() - t1
@@ -342,10 +345,10 @@ def run_surrogate(args, steps):
 
 
 def bench_surrogate(args):
-    res = run_surrogate(args, args.steps)
+    res = run_surrogate(args, args.steps, args.warmup)
     return {
         "metric": "bead-steps/s", "value": res["fp32_model"]["bead_steps_per_s"], "unit": "bead-steps/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": 100, "ms_per_step": res["fp32_model"]["ms_per_step"],
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["fp32_model"]["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 pairs, bf16 far-field network (f32 accumulate), f64 integrate", "data": "synthetic",
         "config": {"workload": res["workload"] + "; far field from the network", "subunits": args.sur_subunits,
@@ -406,7 +409,7 @@ def bench_md(args):
                         "fp32_pair_energy_drift_per_bead": f32["e_end"] - f32["e_start"]}
         for r in (bd, f32):
             r["system"].close()
-        sur = run_surrogate(args, args.sur_steps)
+        sur = run_surrogate(args, args.sur_steps, 100)
         out["extra"]["surrogate_5000"] = {
             "workload": sur["workload"],
             "fp64_exact_bead_steps_per_s": sur["fp64_exact"]["bead_steps_per_s"],
