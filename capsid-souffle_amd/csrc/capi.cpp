@@ -569,6 +569,7 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
         P.Qb[p.q_chain - 1] = 0; P.dof[p.q_chain - 1] = dof3n;
       }
     }
+    for (int j = 0; j < kMaxChain; j++) P.inv_Qb[j] = P.Qb[j] != 0 ? 1.0 / P.Qb[j] : 0.0;
     for (int s = 0; s < n_subunits[r]; s++) sub_rep[s0 + s] = r;
     s0 += n_subunits[r];
   }

@@ -97,16 +97,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Nose-Hoover chain variable j (src/functions.cpp:33-58).
-__device__ void chain_xi(int j, double* xi, const double* Qb, const double* dof, double T, double dt,
-                         double ke) {
+// invQ[j] = 1 / Qb[j], formed on the host (the same correctly rounded value as the division).
+__device__ void chain_xi(int j, double* xi, const double* Qb, const double* invQ, const double* dof, double T,
+                         double dt, double ke) {
   if (Qb[j] == 0) return;
   if (j != 0) {
     xi[j] = xi[j] * exp(-0.5 * dt * xi[j + 1]) +
-            0.5 * dt * (1.0 / Qb[j]) * (Qb[j - 1] * xi[j - 1] * xi[j - 1] - dof[j] * 1.0 * T) *
+            0.5 * dt * invQ[j] * (Qb[j - 1] * xi[j - 1] * xi[j - 1] - dof[j] * 1.0 * T) *
                 exp(-0.25 * dt * xi[j + 1]);
   } else {
     xi[j] = xi[j] * exp(-0.5 * dt * xi[j + 1]) +
-            0.5 * dt * (1.0 / Qb[j]) * (2 * ke - dof[j] * 1.0 * T) * exp(-0.25 * dt * xi[j + 1]);
+            0.5 * dt * invQ[j] * (2 * ke - dof[j] * 1.0 * T) * exp(-0.25 * dt * xi[j + 1]);
   }
 }
 
@@ -1543,14 +1544,24 @@ constexpr int kFinThreads = 1024;
 __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
   __shared__ double wred[kFinThreads / 64][kPartSlots];
   __shared__ double tot[kPartSlots];
+  __shared__ double cQ[kMaxChain], cinvQ[kMaxChain], cdof[kMaxChain], cxi[kMaxChain], ceta[kMaxChain];
   const int r = blockIdx.x, tid = threadIdx.x;
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
-  // wave 0's lanes take the chain state now (lane j: bath j), so the loads overlap the sums
+  // everything the serial tail reads is requested now, so its latency overlaps the sums: the
+  // chain constants and state to LDS (lane j: bath j), the scalars to registers
   const int q = P.q_chain, brownian = P.brownian;
   const bool lists = init || S.rebuild;
   double xi_l = 0.0;
-  if (tid < kMaxChain && !init && !brownian) xi_l = S.xi_n[tid];
+  if (tid < kMaxChain) {
+    cQ[tid] = P.Qb[tid]; cinvQ[tid] = P.inv_Qb[tid]; cdof[tid] = P.dof[tid];
+    const double xn = init ? 0.0 : S.xi_n[tid];
+    cxi[tid] = xn; ceta[tid] = init ? 0.0 : S.eta_n[tid];
+    if (!init && !brownian) xi_l = xn;
+  }
+  const double dt = P.dt, T = P.T, inv_nb = P.inv_nb;
+  const uint32_t seed = P.seed, mult6n = P.mult6n, rng0 = S.rng;
+  const long long step0 = S.step, rebuilds0 = S.rebuilds;
   // slot-major partials: a wave reads 64 consecutive subunits of one slot per load; each thread
   // adds its subunits tid, tid + kFinThreads, ... in that order, with the loads of kFinBatch of
   // them in flight together. Slots 4-6 (BD noise) and 7-8 (list sizes, written at list builds)
@@ -1592,7 +1603,6 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
   __syncthreads();
   if (tid >= 64) return;
   const double ke = tot[0];
-  const double dt = P.dt, T = P.T;
   if (!brownian) {
     // forward chain of this step (src/functions.cpp:33-58 in bath order): bath j's update
     // reads xi[j+1] before that bath is updated, so lane j + 1 evaluates both exponentials of
@@ -1605,12 +1615,15 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
       E2[j] = __shfl(e2, j + 1, 64);
     }
     if (tid != 0) return;
-    double Qb[kMaxChain], dof[kMaxChain], xi[kMaxChain + 1], eta[kMaxChain];
+    // bath constants are read from LDS where used; the chain state is in registers
+    const double* Qb = cQ;
+    const double* invQ = cinvQ;
+    const double* dof = cdof;
+    double xi[kMaxChain + 1], eta[kMaxChain];
 #pragma unroll
     for (int j = 0; j < kMaxChain; j++) {
-      Qb[j] = P.Qb[j]; dof[j] = P.dof[j];
-      xi[j] = init ? 0.0 : S.xi_n[j];
-      eta[j] = init ? 0.0 : S.eta_n[j];
+      xi[j] = cxi[j];
+      eta[j] = ceta[j];
     }
     xi[kMaxChain] = 0.0;
     if (!init) {
@@ -1622,7 +1635,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
       for (int j = 0; j < kMaxChain; j++) {
         if (j >= q || Qb[j] == 0) continue;
         const double g = j == 0 ? 2 * ke - dof[0] * 1.0 * T : Qb[j - 1] * xi[j - 1] * xi[j - 1] - dof[j] * 1.0 * T;
-        xi[j] = xi[j] * E1[j] + 0.5 * dt * (1.0 / Qb[j]) * g * E2[j];
+        xi[j] = xi[j] * E1[j] + 0.5 * dt * invQ[j] * g * E2[j];
       }
     }
 #pragma unroll
@@ -1630,7 +1643,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
     // first thermostat half of the next step: reverse chain, eta update
 #pragma unroll
     for (int j = kMaxChain - 1; j >= 0; j--)
-      if (j < q) chain_xi(j, xi, Qb, dof, T, dt, ke);
+      if (j < q) chain_xi(j, xi, Qb, invQ, dof, T, dt, ke);
 #pragma unroll
     for (int j = 0; j < kMaxChain; j++)
       if (j < q && Qb[j] != 0) eta[j] = eta[j] + 0.5 * dt * xi[j];
@@ -1646,16 +1659,17 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
     S.mean_v[0] = S.mean_v[1] = S.mean_v[2] = 0.0;
     S.vcom = 0.0;
   } else {
-    S.mean_v[0] = tot[1] * P.inv_nb; S.mean_v[1] = tot[2] * P.inv_nb; S.mean_v[2] = tot[3] * P.inv_nb;
-    S.vcom = sqrt(S.mean_v[0] * S.mean_v[0] + S.mean_v[1] * S.mean_v[1] + S.mean_v[2] * S.mean_v[2]);
-    S.step += 1;
+    const double mx = tot[1] * inv_nb, my = tot[2] * inv_nb, mz = tot[3] * inv_nb;
+    S.mean_v[0] = mx; S.mean_v[1] = my; S.mean_v[2] = mz;
+    S.vcom = sqrt(mx * mx + my * my + mz * mz);
+    S.step = step0 + 1;
   }
   if (brownian) {
-    S.rng = init ? P.seed : lcg_mulmod(S.rng, P.mult6n);
-    S.mean_fran[0] = tot[4] * P.inv_nb; S.mean_fran[1] = tot[5] * P.inv_nb; S.mean_fran[2] = tot[6] * P.inv_nb;
+    S.rng = init ? seed : lcg_mulmod(rng0, mult6n);
+    S.mean_fran[0] = tot[4] * inv_nb; S.mean_fran[1] = tot[5] * inv_nb; S.mean_fran[2] = tot[6] * inv_nb;
   }
   if (lists) {
-    if (!init) S.rebuilds += 1;
+    if (!init) S.rebuilds = rebuilds0 + 1;
     S.lj_entries = (long long)tot[7];
     S.q_entries = (long long)tot[8];
   }

@@ -120,6 +120,7 @@ struct RepParams {
   int grid_ok;                   // 1 if the local build grid fits in half a box
   double inv_nb;                 // 1.0 / beads of the replica
   double Qb[kMaxChain];          // bath masses
+  double inv_Qb[kMaxChain];      // 1 / Qb (0 where Qb = 0)
   double dof[kMaxChain];         // bath degrees of freedom (as double)
   int s0, ns, b0, nb, q0, nqr;   // subunit / bead / charged-slot ranges
   int q_chain, brownian, pair_fp32, split;  // split: uncut Yukawa as near (lists) + far field
