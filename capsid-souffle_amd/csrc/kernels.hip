@@ -90,6 +90,10 @@ __device__ __forceinline__ void bd_draws(uint32_t state, int b, double& ux, doub
   uz = canonical_uniform(g[4], g[5]);
 }
 
+// Replica of subunit s; a handle holding one system skips the table lookup, so that the
+// replica's parameters are requested without waiting for it.
+__device__ __forceinline__ int rep_of(const Dev& d, int s) { return d.R == 1 ? 0 : d.sub_rep[s]; }
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -132,7 +136,7 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
   const int np = d.topo.np;
   const int l = local_of(b, np, 1.0f / (float)np);
   const int s = (int)((float)(b - l) * (1.0f / (float)np) + 0.5f);
-  const int r = d.sub_rep[s];
+  const int r = rep_of(d, s);
   double4 p = d.pos[b];
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
@@ -325,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
   double4* stage = reinterpret_cast<double4*>(rec);  // exact fallback and final sums reuse the records
 
   const int s = blockIdx.x, tid = threadIdx.x;
-  const int r = d.sub_rep[s];
+  const int r = rep_of(d, s);
   RepState& S = d.rs[r];
   if (!force_rebuild && S.rebuild == 0) return;
   const RepParams& P = d.rp[r];
@@ -617,7 +621,7 @@ __device__ __forceinline__ void fill_grid(bool in, int c, const float4& rec, int
 __global__ __launch_bounds__(kBlock) void k_cell_fill(Dev d, int force_rebuild) {
   __shared__ int key[kFillSlots], cntl[kFillSlots], base[kFillSlots];
   const int s = blockIdx.x, tid = threadIdx.x;
-  const int r = d.sub_rep[s];
+  const int r = rep_of(d, s);
   RepState& S = d.rs[r];
   if (!force_rebuild && S.rebuild == 0) return;
   const RepParams& P = d.rp[r];
@@ -924,7 +928,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   __shared__ unsigned long long tmask[kMaxTypes];
   __shared__ double red[4];
   const int s = blockIdx.x, tid = threadIdx.x;
-  const int r = d.sub_rep[s];
+  const int r = rep_of(d, s);
   RepState& S = d.rs[r];
   if (!force_rebuild && S.rebuild == 0) return;
   const RepParams& P = d.rp[r];
@@ -1313,7 +1317,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   double* yk_s = qq_s + nq * nq;
 
   const int s = blockIdx.x, tid = threadIdx.x;
-  const int r = d.sub_rep[s];
+  const int r = rep_of(d, s);
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
   const double box = P.box, hbox = P.hbox, kb = P.kb, ks = P.ks, dt = P.dt;
@@ -1680,7 +1684,7 @@ __global__ void k_apply_com(Dev d) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= d.Nb) return;
   const int s = b / d.topo.np;
-  const RepState& S = d.rs[d.sub_rep[s]];
+  const RepState& S = d.rs[rep_of(d, s)];
   d.vx[b] -= S.mean_v[0]; d.vy[b] -= S.mean_v[1]; d.vz[b] -= S.mean_v[2];
 }
 __global__ void k_clear_com(Dev d) {
@@ -1706,7 +1710,7 @@ __global__ __launch_bounds__(kFarThreads) void k_far_exact(Dev d, int energy) {
   __shared__ double red[kFarThreads / 64][kFarQ * 4];
   __shared__ double qq_s[kFarQ * kFarQ], yk_s[kFarQ * kFarQ];
   const int s = blockIdx.x, tid = threadIdx.x;
-  const int r = d.sub_rep[s];
+  const int r = rep_of(d, s);
   const RepParams& P = d.rp[r];
   const int nq = d.topo.nq;
   const double box = P.box, hbox = P.hbox, kappa = P.kappa;
@@ -1767,7 +1771,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
   double4* own = reinterpret_cast<double4*>(smem);
   double* he = reinterpret_cast<double*>(own + np);
   const int s = blockIdx.x, tid = threadIdx.x;
-  const int r = d.sub_rep[s];
+  const int r = rep_of(d, s);
   const RepParams& P = d.rp[r];
   const double box = P.box, hbox = P.hbox;
   const double4 cs = d.centres[s];
