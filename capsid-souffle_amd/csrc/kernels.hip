@@ -1226,9 +1226,9 @@ __device__ __forceinline__ PT lj_term(PT r2, bool att, int tj, PT si, PT sj, PT 
     eps = (PT)1;
   }
   if (!on) return (PT)0;
-  const PT r6 = r2 * r2 * r2;
-  const PT sr6 = s6 / r6;
-  return (PT)48 * eps * (sr6 * (sr6 - (PT)0.5)) * ((PT)1 / r2);
+  const PT ir2 = (PT)1 / r2;  // one division; (s/r)^6 = s^6 (1/r^2)^3
+  const PT sr6 = s6 * (ir2 * ir2 * ir2);
+  return (PT)48 * eps * (sr6 * (sr6 - (PT)0.5)) * ir2;
 }
 
 // Screened Coulomb term (src/forces.cpp:100-108): q_i q_j lb exp(-k r)/r^2 (k + 1/r) Y_ij below
@@ -1236,8 +1236,8 @@ __device__ __forceinline__ PT lj_term(PT r2, bool att, int tj, PT si, PT sj, PT 
 template <typename PT>
 __device__ __forceinline__ PT yk_term(PT r2, PT qq, PT yk, PT kappa, PT ecut_el2) {
   if (!(r2 < ecut_el2)) return (PT)0;
-  const PT rr = sqrt(r2);
-  return (qq * exp(-kappa * rr) / r2) * (kappa + (PT)1 / rr) * yk;
+  const PT ir = rsqrt(r2), rr = r2 * ir;  // no division: 1/r = ir, 1/r^2 = ir^2
+  return (qq * exp(-kappa * rr) * (ir * ir)) * (kappa + ir) * yk;
 }
 
 // What one pair-list entry of a force workgroup needs besides its neighbour's position: the
