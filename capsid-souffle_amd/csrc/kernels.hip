@@ -1350,6 +1350,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   const int4 hrow1 = reinterpret_cast<const int4*>(d.topo.h_slot)[2 * lt + 1];
   const double m = d.topo.mass[lt];
   const int4 hg0 = reinterpret_cast<const int4*>(d.topo.hinge)[tid < nh ? tid : 0];
+  const int4 hg1 = reinterpret_cast<const int4*>(d.topo.hinge)[tid + kBlock < nh ? tid + kBlock : 0];
   double vx = 0, vy = 0, vz = 0;
   if (active) { vx = d.vx[gi]; vy = d.vy[gi]; vz = d.vz[gi]; }
   // offsets of each bead's entries in the subunit's concatenated lists (LJ entries, then
@@ -1378,7 +1379,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     for (int e = off_i; e < min(off_i + n_i, c1_first); e++) owner_s[e] = (unsigned char)tid;
   __syncthreads();
   static_assert(kChunk % kBlock == 0 && kPre >= 1, "tile of a multiple of the workgroup size");
-  double4 pre0 = make_double4(0, 0, 0, 0), pre1 = pre0;  // no indexed array
+  // list indices now, the neighbour positions once the hinges are done (the index loads are
+  // not waited for before the bonded terms start)
+  int pj0 = -1, pj1 = -1;
 #pragma unroll
   for (int u = 0; u < kPre; u++) {
     const int e = tid + u * kBlock;
@@ -1388,18 +1391,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
       const int nl = nlj_s[i];
       const int j = (k < nl) ? d.lj_list[(size_t)k * d.Nb + s * np + i]
                              : d.q_list[(size_t)(k - nl) * d.Nq + (s * nq + tb.qslot[i])];
-      const double4 v = d.pos[j];
-      if (u == 0) pre0 = v; else pre1 = v;
+      if (u == 0) pj0 = j; else pj1 = j;
     }
   }
-  // bending hinges -> LDS
+  // bending hinges -> LDS (the rows of a thread's first two hinges were loaded with the tables)
   for (int h = tid; h < nh; h += kBlock) {
-    const int4 hg = (h == tid) ? hg0 : reinterpret_cast<const int4*>(d.topo.hinge)[h];
+    const int4 hg = (h == tid) ? hg0 : (h == tid + kBlock) ? hg1 : reinterpret_cast<const int4*>(d.topo.hinge)[h];
     double hf[12];
     hinge_forces(own, hg, box, hbox, kb, hf);
 #pragma unroll
     for (int k = 0; k < 9; k++) hc[9 * h + k] = hf[3 + k];
   }
+  double4 pre0 = make_double4(0, 0, 0, 0), pre1 = pre0;  // no indexed array
+  if (pj0 >= 0) pre0 = d.pos[pj0];
+  if (kPre > 1 && pj1 >= 0) pre1 = d.pos[pj1];
   // stretching (src/bead.cpp:17-23) over the bead's edge row
   double sx = 0, sy = 0, sz = 0;
   double4 pi = make_double4(0, 0, 0, 0);
