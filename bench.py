@@ -79,7 +79,7 @@ def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
         # the force kernel's pair tile by the library's autotuner at the fixed skin, after the
         # warm-up so that the trials see clusters forming (trials run from a saved state that is
         # restored; forces do not depend on the tile)
-        tune, _ = s.autotune(skins=[s.skin(0)], tiles=[128, 256, 512], trial_steps=200)
+        tune, _ = s.autotune(skins=[s.skin(0)], tiles=[128, 256, 384, 512], trial_steps=200)
     st0 = s.stats()
     t0 = time.perf_counter()
     s.step(args.steps)
@@ -252,7 +252,7 @@ def bench_replicas(args):
         "kernels_ms": {"k_integrate": st["k_integrate_ms"], "k_rebuild": st["k_rebuild_ms"],
                        "k_force": st["k_force_ms"], "k_finalize": st["k_reduce_ms"]},
         "autotune": {"skin": tune["skin"], "tile": tune["tile"], "trial_ms_per_step": trial_ms.round(4).tolist(),
-                     "skins": [0.2, 0.3, 0.5, 0.8, 1.2], "tiles": [128, 256, 512]},
+                     "skins": [0.2, 0.3, 0.5, 0.8, 1.2], "tiles": [128, 256, 384, 512]},
         "single_run_check": checked,
     }
     if not args.no_cpu:

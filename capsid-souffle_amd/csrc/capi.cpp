@@ -475,7 +475,7 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
   if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed (no GPU?)");
   Dev& d = h->d;
   d.Nb = h->Nb; d.Ns = h->Ns; d.Nq = h->Nq; d.R = R;
-  d.tile = 256;
+  d.tile = 384;  // the autotuner may choose another
   d.split = split;
   // template tables
   TopoDev& td = d.topo;
@@ -821,7 +821,7 @@ int sf_bd_uniforms(sf_handle* h, int replica, int64_t step, double* out) {
 
 int sf_set_tile(sf_handle* h, int tile) {
   if (!h) { set_error("null handle"); return SF_E_ARG; }
-  if (!tile_supported(tile)) { set_error("tile must be 128, 256 or 512"); return SF_E_ARG; }
+  if (!tile_supported(tile)) { set_error("tile must be 128, 256, 384 or 512"); return SF_E_ARG; }
   h->d.tile = tile;
   return SF_OK;
 }
@@ -851,10 +851,10 @@ int sf_get_skin(sf_handle* h, int replica, double* skin) {
 int sf_autotune(sf_handle* h, const sf_tune_opts* opt, sf_tune_result* res, double* trial_ms) {
   if (!h || !res) { set_error("null argument"); return SF_E_ARG; }
   static const double kSkins[] = {0.2, 0.3, 0.5, 0.8, 1.2};
-  static const int kTiles[] = {128, 256, 512};
+  static const int kTiles[] = {128, 256, 384, 512};
   const double* skins = kSkins;
   const int* tiles = kTiles;
-  int ns = 5, nt = 3, trial = 200;
+  int ns = (int)(sizeof(kSkins) / sizeof(kSkins[0])), nt = (int)(sizeof(kTiles) / sizeof(kTiles[0])), trial = 200;
   if (opt) {
     if (opt->n_skins > 0) { if (!opt->skins) { set_error("skins is null"); return SF_E_ARG; } skins = opt->skins; ns = opt->n_skins; }
     if (opt->n_tiles > 0) { if (!opt->tiles) { set_error("tiles is null"); return SF_E_ARG; } tiles = opt->tiles; nt = opt->n_tiles; }
@@ -863,7 +863,7 @@ int sf_autotune(sf_handle* h, const sf_tune_opts* opt, sf_tune_result* res, doub
   for (int i = 0; i < ns; i++)
     if (!(skins[i] > 0)) { set_error("candidate skins must be positive"); return SF_E_ARG; }
   for (int j = 0; j < nt; j++)
-    if (!tile_supported(tiles[j])) { set_error("candidate tiles must be 128, 256 or 512"); return SF_E_ARG; }
+    if (!tile_supported(tiles[j])) { set_error("candidate tiles must be 128, 256, 384 or 512"); return SF_E_ARG; }
   // save the dynamic state; every trial starts from it and the caller gets it back
   const size_t nb = h->Nb;
   double4* a_pos = nullptr; double* a_v = nullptr; double* a_f = nullptr; double* a_g = nullptr;

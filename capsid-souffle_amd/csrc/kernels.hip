@@ -1304,7 +1304,6 @@ template <int MODE, typename PT, int kChunk>
 // Four waves per SIMD is what the LDS allows (8 workgroups per CU): up to 128 VGPRs.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_force(Dev d, int init) {
   extern __shared__ double smem[];
-  __shared__ double red[2][kPartStride];
   __shared__ BeadLds tb;
   __shared__ int off_s[kBlock + 1];
   __shared__ int nlj_s[kBlock];
@@ -1313,6 +1312,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   double4* own = reinterpret_cast<double4*>(smem);
   double* hc = reinterpret_cast<double*>(own + np);          // [nh][9] forces on hinge beads 1-3
   PT* pf = reinterpret_cast<PT*>(hc);                        // [3][kChunk], after the bending sums
+  double (*red)[kPartStride] = reinterpret_cast<double (*)[kPartStride]>(hc);  // epilogue sums (tile free)
   double* qq_s = hc + max(9 * nh, 3 * kChunk);
   double* yk_s = qq_s + nq * nq;
 
@@ -1914,7 +1914,7 @@ int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st) {
   return check_launch();
 }
 
-bool tile_supported(int tile) { return tile == 128 || tile == 256 || tile == 512; }
+bool tile_supported(int tile) { return tile == 128 || tile == 256 || tile == 384 || tile == 512; }
 
 template <int CH>
 static void launch_force_tile(const Dev& d, int mode, int init, int fp32, size_t sm, hipStream_t st) {
@@ -1935,6 +1935,7 @@ int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st) {
   if (d.tile == 128) launch_force_tile<128>(d, mode, init, fp32, sm, st);
   else if (d.tile == 512) launch_force_tile<512>(d, mode, init, fp32, sm, st);
   else if (d.tile == 256) launch_force_tile<256>(d, mode, init, fp32, sm, st);
+  else if (d.tile == 384) launch_force_tile<384>(d, mode, init, fp32, sm, st);
   else { set_error("unsupported k_force tile"); return SF_E_ARG; }
   return check_launch();
 }

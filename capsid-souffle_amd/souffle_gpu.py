@@ -283,7 +283,7 @@ class System:
         """Runs the library's autotuner; returns (result dict, trial ms/step [n_skins][n_tiles])."""
         o = SfTuneOpts(trial_steps=int(trial_steps))
         sk = np.ascontiguousarray(skins if skins is not None else [0.2, 0.3, 0.5, 0.8, 1.2], dtype=np.float64)
-        tl = np.ascontiguousarray(tiles if tiles is not None else [128, 256, 512], dtype=np.int32)
+        tl = np.ascontiguousarray(tiles if tiles is not None else [128, 256, 384, 512], dtype=np.int32)
         o.n_skins, o.skins = len(sk), sk.ctypes.data_as(_D)
         o.n_tiles, o.tiles = len(tl), tl.ctypes.data_as(_I)
         res = SfTuneResult()

@@ -187,14 +187,14 @@ int sf_set_skin(sf_handle* h, int replica, double skin);
 int sf_get_skin(sf_handle* h, int replica, double* skin);
 
 /* Pair-gather tile of the force kernel: list entries a subunit workgroup evaluates per round
- * (128, 256 or 512). Results are the same bit for bit for every tile. */
+ * (128, 256, 384 or 512). Results are the same bit for bit for every tile. */
 int sf_set_tile(sf_handle* h, int tile);
 
 typedef struct {
   int trial_steps;       /* timed steps per candidate; <= 0: 200 */
   int n_skins;           /* candidate skins; 0: {0.2, 0.3, 0.5, 0.8, 1.2} */
   const double* skins;
-  int n_tiles;           /* candidate tiles; 0: {128, 256, 512} */
+  int n_tiles;           /* candidate tiles; 0: {128, 256, 384, 512} */
   const int* tiles;
 } sf_tune_opts;
 

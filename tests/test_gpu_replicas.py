@@ -35,7 +35,9 @@ def test_skin_and_tile_leave_trajectory_unchanged():
     assert b.skin() == 1.0
     b.set_skin(0.15)
     b.set_tile(512)
-    b.step(50)
+    b.step(25)
+    b.set_tile(256)  # the first 50 steps ran at the default tile, 384
+    b.step(25)
     bp, bv = b.get_state()
     assert np.array_equal(ap, bp) and np.array_equal(av, bv)
     with pytest.raises(sg.SfError):
