@@ -541,7 +541,7 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
       }
     for (int l = 0; l < np; l++) amp[l] = brown ? std::sqrt((t->mass[l] * 24.0) / (p.damp * p.dt)) : 0.0;
     RepParams& P = h->rp[r];
-    P.box = p.box; P.hbox = p.box * 0.5;
+    P.box = p.box; P.hbox = p.box * 0.5; P.ibox = 1.0 / p.box;
     P.ecut = p.ecut; P.ecut2 = p.ecut * p.ecut;
     P.ecut_el = p.ecut_el; P.ecut_el2 = p.ecut_el * p.ecut_el;
     P.kappa = p.kappa; P.lb = p.lb; P.elj_att = p.elj_att;

@@ -27,6 +27,13 @@ __device__ __forceinline__ double min_img(double v, double box, double hbox) {
   else if (v < -hbox) v += box;
   return v;
 }
+// The same single shift for a difference whose true separation is far below half a box (beads of
+// one subunit, list neighbours, a bead and its position at the last list build): the shift is
+// the nearest multiple of the box, formed without a branch. Identical to min_img for such
+// differences (they are never within rounding of half a box).
+__device__ __forceinline__ double min_img_near(double v, double box, double ibox) {
+  return fma(-rint(v * ibox), box, v);
+}
 __device__ __forceinline__ float min_imgf(float v, float box, float hbox) {
   if (v > hbox) v -= box;
   else if (v < -hbox) v += box;
@@ -161,8 +168,9 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
   if (p.z > hbox) p.z = p.z - box; else if (p.z < -hbox) p.z = p.z + box;
   d.pos[b] = p;
   d.vx[b] = vx; d.vy[b] = vy; d.vz[b] = vz;
-  const double ddx = min_img(p.x - q.x, box, hbox), ddy = min_img(p.y - q.y, box, hbox),
-               ddz = min_img(p.z - q.z, box, hbox);
+  const double ibox = P.ibox;
+  const double ddx = min_img_near(p.x - q.x, box, ibox), ddy = min_img_near(p.y - q.y, box, ibox),
+               ddz = min_img_near(p.z - q.z, box, ibox);
   if (ddx * ddx + ddy * ddy + ddz * ddz > half_skin2) S.rebuild = 1;  // same value from any thread
   if (l == d.topo.centre) d.centres[s] = p;
   const int qs = d.topo.qslot[l];
@@ -1168,15 +1176,15 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
 // v = e x b, c = u.v/(|u||v|): dc/du = g = (v/|v| - c u/|u|)/|u|, dc/dv = h likewise, and
 // dc/dx2 = g x e, dc/dx3 = h x e, dc/dx1 = a x g + b x h, dc/dx0 = -(sum).
 // Writes the 4 bead forces (12 doubles) and returns cos t.
-__device__ __forceinline__ double hinge_forces(const double4* own, int4 hg, double box, double hbox, double Kb,
+__device__ __forceinline__ double hinge_forces(const double4* own, int4 hg, double box, double ibox, double Kb,
                                                double* out) {
   const double4 p0 = own[hg.x], p1 = own[hg.y], p2 = own[hg.z], p3 = own[hg.w];
-  const double ex = min_img(p1.x - p0.x, box, hbox), ey = min_img(p1.y - p0.y, box, hbox),
-               ez = min_img(p1.z - p0.z, box, hbox);
-  const double ax = min_img(p2.x - p0.x, box, hbox), ay = min_img(p2.y - p0.y, box, hbox),
-               az = min_img(p2.z - p0.z, box, hbox);
-  const double bx = min_img(p3.x - p0.x, box, hbox), by = min_img(p3.y - p0.y, box, hbox),
-               bz = min_img(p3.z - p0.z, box, hbox);
+  const double ex = min_img_near(p1.x - p0.x, box, ibox), ey = min_img_near(p1.y - p0.y, box, ibox),
+               ez = min_img_near(p1.z - p0.z, box, ibox);
+  const double ax = min_img_near(p2.x - p0.x, box, ibox), ay = min_img_near(p2.y - p0.y, box, ibox),
+               az = min_img_near(p2.z - p0.z, box, ibox);
+  const double bx = min_img_near(p3.x - p0.x, box, ibox), by = min_img_near(p3.y - p0.y, box, ibox),
+               bz = min_img_near(p3.z - p0.z, box, ibox);
   const double ux = ey * az - ez * ay, uy = ez * ax - ex * az, uz = ex * ay - ey * ax;
   const double vx = ey * bz - ez * by, vy = ez * bx - ex * bz, vz = ex * by - ey * bx;
   const double iu = rsqrt(ux * ux + uy * uy + uz * uz), iv = rsqrt(vx * vx + vy * vy + vz * vz);
@@ -1254,7 +1262,7 @@ struct PairCtx {
   const double* yk;
   const double4* centres;
   double4 cs;
-  double box, hbox, far_rf2;
+  double box, hbox, ibox, far_rf2;
   PT ecut2, elj, kappa, ecut_el2;
   int nq, split;
 };
@@ -1268,8 +1276,8 @@ __device__ __forceinline__ void pair_entry(int le, int c0, double4 pj, PairCtx<P
   const int k = c0 + le - c.off[i];
   const int nl = c.nlj[i];
   const double4 p0 = c.own[i];
-  const PT dx = (PT)min_img(p0.x - pj.x, c.box, c.hbox), dy = (PT)min_img(p0.y - pj.y, c.box, c.hbox),
-           dz = (PT)min_img(p0.z - pj.z, c.box, c.hbox);
+  const PT dx = (PT)min_img_near(p0.x - pj.x, c.box, c.ibox), dy = (PT)min_img_near(p0.y - pj.y, c.box, c.ibox),
+           dz = (PT)min_img_near(p0.z - pj.z, c.box, c.ibox);
   const PT r2 = dx * dx + dy * dy + dz * dz;
   const int lj = tag_loc(pj.w);
   PT f;
@@ -1320,7 +1328,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   const int r = rep_of(d, s);
   const RepParams& P = d.rp[r];
   RepState& S = d.rs[r];
-  const double box = P.box, hbox = P.hbox, kb = P.kb, ks = P.ks, dt = P.dt;
+  const double box = P.box, hbox = P.hbox, ibox = P.ibox, kb = P.kb, ks = P.ks, dt = P.dt;
   const double4 cs = d.centres[s];
   const bool active = tid < np;
   const int gi = s * np + tid;
@@ -1398,7 +1406,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   for (int h = tid; h < nh; h += kBlock) {
     const int4 hg = (h == tid) ? hg0 : (h == tid + kBlock) ? hg1 : reinterpret_cast<const int4*>(d.topo.hinge)[h];
     double hf[12];
-    hinge_forces(own, hg, box, hbox, kb, hf);
+    hinge_forces(own, hg, box, ibox, kb, hf);
 #pragma unroll
     for (int k = 0; k < 9; k++) hc[9 * h + k] = hf[3 + k];
   }
@@ -1416,8 +1424,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
       const int o = ((k < 4 ? erow.x : erow.y) >> (8 * (k & 3))) & 0xff;
       const double l0 = (k & 1) ? len[k >> 1].y : len[k >> 1].x;
       const double4 po = own[o];
-      const double dx = min_img(pi.x - po.x, box, hbox), dy = min_img(pi.y - po.y, box, hbox),
-                   dz = min_img(pi.z - po.z, box, hbox);
+      const double dx = min_img_near(pi.x - po.x, box, ibox), dy = min_img_near(pi.y - po.y, box, ibox),
+                   dz = min_img_near(pi.z - po.z, box, ibox);
       const double L2 = dx * dx + dy * dy + dz * dz, iL = rsqrt(L2), L = L2 * iL;
       const double c = (l0 - L) * ks * iL;
       sx += dx * c; sy += dy * c; sz += dz * c;
@@ -1448,7 +1456,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   {
     PairCtx<PT> ctx;
     ctx.owner = owner_s; ctx.off = off_s; ctx.nlj = nlj_s; ctx.tb = &tb; ctx.own = own; ctx.qq = qq_s; ctx.yk = yk_s;
-    ctx.centres = d.centres; ctx.cs = cs; ctx.box = box; ctx.hbox = hbox; ctx.far_rf2 = P.far_rf2;
+    ctx.centres = d.centres; ctx.cs = cs; ctx.box = box; ctx.hbox = hbox; ctx.ibox = ibox; ctx.far_rf2 = P.far_rf2;
     ctx.ecut2 = (PT)P.ecut2; ctx.elj = (PT)P.elj_att; ctx.kappa = (PT)P.kappa; ctx.ecut_el2 = (PT)P.ecut_el2;
     ctx.nq = nq; ctx.split = P.split;
     for (int c0 = 0; c0 < total; c0 += kChunk) {
@@ -1787,7 +1795,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
   for (int h = tid; h < nh; h += kBlock) {
     double tmp[12];
     const int4 hg = reinterpret_cast<const int4*>(d.topo.hinge)[h];
-    const double c = hinge_forces(own, hg, box, hbox, P.kb, tmp);
+    const double c = hinge_forces(own, hg, box, P.ibox, P.kb, tmp);
     he[h] = P.kb * 0.25 * (1 - fabs(c));  // src/edge.cpp:64-66 after the flip: n0.n1 = |cos|
   }
   __syncthreads();

@@ -107,7 +107,7 @@ struct TopoDev {
 
 // Per-replica constants.
 struct RepParams {
-  double box, hbox;
+  double box, hbox, ibox;        // ibox = 1 / box
   double ecut, ecut2, ecut_el, ecut_el2, kappa, lb, elj_att;
   double ks, kb, dt, damp, T;
   double skin, half_skin2;
