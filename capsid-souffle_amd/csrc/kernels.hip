@@ -34,11 +34,10 @@ __device__ __forceinline__ double min_img(double v, double box, double hbox) {
 __device__ __forceinline__ double min_img_near(double v, double box, double ibox) {
   return fma(-rint(v * ibox), box, v);
 }
-__device__ __forceinline__ float min_imgf(float v, float box, float hbox) {
-  if (v > hbox) v -= box;
-  else if (v < -hbox) v += box;
-  return v;
-}
+// Branch-free single shift in FP32 for the list builds: the same value as the conditional shift
+// for a separation far below half a box; elsewhere both give a separation of about half a box,
+// so a test against a list radius (< half a box) decides the same.
+__device__ __forceinline__ float min_imgf_near(float v, float box, float ibox) { return fmaf(-rintf(v * ibox), box, v); }
 
 // Template index of bead j (subunit-major numbering) by a float reciprocal and an exact fix-up.
 __device__ __forceinline__ int local_of(int j, int np, float inv_np) {
@@ -343,7 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
   const RepParams& P = d.rp[r];
   const int np = d.topo.np, nq = d.topo.nq;
   const double box = P.box, hbox = P.hbox;
-  const float boxf = (float)box, hboxf = (float)hbox;
+  const float boxf = (float)box, iboxf = (float)P.ibox;
   const double4 cs = d.centres[s];
   const bool active = tid < np;
   const int gi = s * np + tid;
@@ -445,8 +444,8 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
             const int j = __float_as_int(rj.w);
             const int lj = local_of(j, np, inv_np);
             const int tj = ttype[lj];
-            const float dx = min_imgf(ix - rj.x, boxf, hboxf), dy = min_imgf(iy - rj.y, boxf, hboxf),
-                        dz = min_imgf(iz - rj.z, boxf, hboxf);
+            const float dx = min_imgf_near(ix - rj.x, boxf, iboxf), dy = min_imgf_near(iy - rj.y, boxf, iboxf),
+                        dz = min_imgf_near(iz - rj.z, boxf, iboxf);
             const float r2 = dx * dx + dy * dy + dz * dz;
             float thr2;
             if ((mask_i >> tj) & 1ull) {
@@ -520,8 +519,8 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
         unsigned msk = 0;
         for (int kq = 0; kq < nq; kq++) {
           const float4 oi = qown[kq];
-          const float dx = min_imgf(oi.x - rj.x, boxf, hboxf), dy = min_imgf(oi.y - rj.y, boxf, hboxf),
-                      dz = min_imgf(oi.z - rj.z, boxf, hboxf);
+          const float dx = min_imgf_near(oi.x - rj.x, boxf, iboxf), dy = min_imgf_near(oi.y - rj.y, boxf, iboxf),
+                      dz = min_imgf_near(oi.z - rj.z, boxf, iboxf);
           if (j != __float_as_int(oi.w) && dx * dx + dy * dy + dz * dz < rq2) msk |= 1u << kq;
         }
         ymask[f] = msk;
@@ -691,10 +690,11 @@ __device__ __forceinline__ int rel_cell(int c, int cc, int g) {
 
 // LJ / WCA list radius^2 test of bead i against candidate record e (FP32, +1e-3 sigma).
 struct LjTest {
-  float ix, iy, iz, si, ratt2, skinf, box, hbox;
+  float ix, iy, iz, si, ratt2, skinf, box, hbox, ibox;
   unsigned long long mask_i;
   __device__ __forceinline__ float r2_to(const float4& e) const {
-    const float dx = min_imgf(ix - e.x, box, hbox), dy = min_imgf(iy - e.y, box, hbox), dz = min_imgf(iz - e.z, box, hbox);
+    const float dx = min_imgf_near(ix - e.x, box, ibox), dy = min_imgf_near(iy - e.y, box, ibox),
+                dz = min_imgf_near(iz - e.z, box, ibox);
     return dx * dx + dy * dy + dz * dz;
   }
   __device__ __forceinline__ bool hit(float r2, int tj, float sj) const {
@@ -899,8 +899,8 @@ __device__ __forceinline__ void cell_query_direct(const Dev& d, const RepParams&
       for (int m = 0; m < n; m++) {
         const float4 e = cell[m];
         const int j = __float_as_int(e.w);
-        const float dx = min_imgf(T.ix - e.x, T.box, T.hbox), dy = min_imgf(T.iy - e.y, T.box, T.hbox),
-                    dz = min_imgf(T.iz - e.z, T.box, T.hbox);
+        const float dx = min_imgf_near(T.ix - e.x, T.box, T.ibox), dy = min_imgf_near(T.iy - e.y, T.box, T.ibox),
+                    dz = min_imgf_near(T.iz - e.z, T.box, T.ibox);
         if (j == gi || !(dx * dx + dy * dy + dz * dz < rq2)) continue;
         if (cnt_q < d.q_cap) d.q_list[(size_t)cnt_q * d.Nq + gq] = j;
         cnt_q++;
@@ -954,7 +954,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   const float cx = wrapf((float)cs.x, box, hbox), cy = wrapf((float)cs.y, box, hbox), cz = wrapf((float)cs.z, box, hbox);
   LjTest T;
   T.ix = wrapf((float)pi.x, box, hbox); T.iy = wrapf((float)pi.y, box, hbox); T.iz = wrapf((float)pi.z, box, hbox);
-  T.box = box; T.hbox = hbox;
+  T.box = box; T.hbox = hbox; T.ibox = (float)P.ibox;
   T.si = active ? (float)d.topo.sigma[tid] : 0.0f;
   T.mask_i = active ? d.topo.att_mask[d.topo.type[tid]] : 0ull;
   const float ratt = (float)(P.ecut + P.skin) + 1e-3f;
@@ -1077,8 +1077,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
         for (int u = 0; u < 8; u++) {
           if (m0 + u >= n) break;
           const int j = __float_as_int(e[u].w);
-          const float dx = min_imgf(qa.x - e[u].x, box, hbox), dy = min_imgf(qa.y - e[u].y, box, hbox),
-                      dz = min_imgf(qa.z - e[u].z, box, hbox);
+          const float dx = min_imgf_near(qa.x - e[u].x, box, T.ibox), dy = min_imgf_near(qa.y - e[u].y, box, T.ibox),
+                      dz = min_imgf_near(qa.z - e[u].z, box, T.ibox);
           if (j == self || !(dx * dx + dy * dy + dz * dz < rq2)) continue;
           const int slot = atomicAdd(&qn[a], 1);
           if (slot < kCqQSlots) qbuf[a * kCqQSlots + slot] = j;
@@ -1127,8 +1127,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
           for (int m = 0; m < n; m++) {
             const float4 e = cell[m];
             const int j = __float_as_int(e.w);
-            const float dx = min_imgf(T.ix - e.x, box, hbox), dy = min_imgf(T.iy - e.y, box, hbox),
-                        dz = min_imgf(T.iz - e.z, box, hbox);
+            const float dx = min_imgf_near(T.ix - e.x, box, T.ibox), dy = min_imgf_near(T.iy - e.y, box, T.ibox),
+                        dz = min_imgf_near(T.iz - e.z, box, T.ibox);
             if (j == gi || !(dx * dx + dy * dy + dz * dz < rq2)) continue;
             if (cnt_q < d.q_cap) d.q_list[(size_t)cnt_q * d.Nq + gq] = j;
             cnt_q++;
