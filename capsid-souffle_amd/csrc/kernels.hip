@@ -22,10 +22,10 @@
 namespace sf {
 
 __device__ __forceinline__ double min_img(double v, double box, double hbox) {
-  // single conditional shift per axis, as src/forces.cpp:85-91 and src/functions.h:42-56
-  if (v > hbox) v -= box;
-  else if (v < -hbox) v += box;
-  return v;
+  // single conditional shift per axis, as src/forces.cpp:85-91 and src/functions.h:42-56, as a
+  // select (v - (-box) is v + box exactly), so that a wave does not branch on it
+  const double sh = v > hbox ? box : (v < -hbox ? -box : 0.0);
+  return v - sh;
 }
 // The same single shift for a difference whose true separation is far below half a box (beads of
 // one subunit, list neighbours, a bead and its position at the last list build): the shift is
@@ -162,9 +162,7 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
     vx = vx + fx * c; vy = vy + fy * c; vz = vz + fz * c;
   }
   p.x = p.x + vx * dt; p.y = p.y + vy * dt; p.z = p.z + vz * dt;
-  if (p.x > hbox) p.x = p.x - box; else if (p.x < -hbox) p.x = p.x + box;
-  if (p.y > hbox) p.y = p.y - box; else if (p.y < -hbox) p.y = p.y + box;
-  if (p.z > hbox) p.z = p.z - box; else if (p.z < -hbox) p.z = p.z + box;
+  p.x = min_img(p.x, box, hbox); p.y = min_img(p.y, box, hbox); p.z = min_img(p.z, box, hbox);  // one wrap
   d.pos[b] = p;
   d.vx[b] = vx; d.vy[b] = vy; d.vz[b] = vz;
   const double ibox = P.ibox;

@@ -28,9 +28,7 @@ int check_launch_far() {
 }
 
 __device__ __forceinline__ double min_img_f(double x, double box, double hbox) {
-  if (x > hbox) return x - box;
-  if (x < -hbox) return x + box;
-  return x;
+  return x - (x > hbox ? box : (x < -hbox ? -box : 0.0));  // single shift as a select
 }
 
 // splitmix64 stream: one independent stream per sample
