@@ -374,8 +374,13 @@ struct sf_handle {
   int g_ccap[2] = {0, 0};
   std::vector<hipEvent_t> events;
 
+  // The executable graph's last launch may still be running on the stream: it is drained before
+  // the graph is destroyed (HIP does not promise to defer the destruction of a running graph).
   void drop_graph() {
-    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (gexec) {
+      (void)hipStreamSynchronize(st);
+      (void)hipGraphExecDestroy(gexec);
+    }
     gexec = nullptr;
   }
   int ensure_events() {
