@@ -980,6 +980,22 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
     const int ox = fx ? ax[0].cc : -ax[0].lo, oy = fy ? ax[1].cc : -ax[1].lo, oz = fz ? ax[2].cc : -ax[2].lo;
     const bool rows = !fx && g > 1;
     const int nseg = g == 1 ? 1 : (rows ? 9 : 27);
+    // the neighbour cells this bead's list radius reaches: on each axis the lower (upper) one
+    // only when the radius box crosses the lower (upper) face of the bead's cell; records in the
+    // other cells fail the radius test anyway (a WCA-only bead, radius ~1.4 sigma against cells of
+    // the largest list radius, needs at most two cells per axis)
+    const float rr = sqrtf(bound2) + 1e-3f, ia = P.inv_a_lj;
+    bool lo[3], hi[3];
+    {
+      const float pc[3] = {T.ix, T.iy, T.iz};
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const float tc = (pc[k] + hbox) * ia, fl = floorf(tc), f = tc - fl;
+        const bool in = fl >= 0.0f && fl < (float)g;  // else the cell was clamped: both neighbours
+        lo[k] = !in || f < rr * ia;
+        hi[k] = !in || 1.0f - f < rr * ia;
+      }
+    }
     bool over = false;
     for (int pass = 0; pass < 2; pass++) {  // pass 1 only when the LDS slots overflowed
       if (pass == 1) {
@@ -992,13 +1008,16 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
           m0 = coff[0]; m1 = m0 + cnum[0];
         } else {
           const int dz = rows ? sg / 3 - 1 : sg / 9 - 1, dy = rows ? sg % 3 - 1 : (sg / 3) % 3 - 1;
+          if ((dz < 0 && !lo[2]) || (dz > 0 && !hi[2]) || (dy < 0 && !lo[1]) || (dy > 0 && !hi[1])) continue;
           const int lz = fz ? wrap_cell(oz + rel[2] + dz, g) : rel[2] + dz + oz;
           const int ly = fy ? wrap_cell(oy + rel[1] + dy, g) : rel[1] + dy + oy;
           if (rows) {
             const int l0 = (lz * sy + ly) * sx + rel[0] - 1 + ox;
-            m0 = coff[l0]; m1 = coff[l0 + 3];
+            m0 = coff[l0 + (lo[0] ? 0 : 1)]; m1 = coff[l0 + (hi[0] ? 3 : 2)];
           } else {
-            const int lc = (lz * sy + ly) * sx + wrap_cell(ox + rel[0] + sg % 3 - 1, g);
+            const int dx = sg % 3 - 1;
+            if ((dx < 0 && !lo[0]) || (dx > 0 && !hi[0])) continue;
+            const int lc = (lz * sy + ly) * sx + wrap_cell(ox + rel[0] + dx, g);
             m0 = coff[lc]; m1 = m0 + cnum[lc];
           }
         }
