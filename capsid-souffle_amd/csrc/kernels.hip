@@ -1561,14 +1561,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     }
   }
   // per-subunit sums, fixed shape: wave shuffles then the two waves in order (slots 7-8 are
-  // the list sizes written by k_rebuild)
+  // the list sizes written by k_rebuild; the BD noise slots 4-6 are not used by NH MD)
+  constexpr int kSums = MODE == MODE_MD ? 4 : 7;
 #pragma unroll
-  for (int k = 0; k < 7; k++) {
+  for (int k = 0; k < kSums; k++) {
     const double w = wave_sum(part[k]);
     if ((tid & 63) == 0) red[tid >> 6][k] = w;
   }
   __syncthreads();
-  if (tid < 7) d.part[tid * (size_t)d.Ns + s] = red[0][tid] + red[1][tid];
+  if (tid < kSums) d.part[tid * (size_t)d.Ns + s] = red[0][tid] + red[1][tid];
 }
 
 // One workgroup per replica: fixed-order sums of the per-subunit partials, then the scalar
