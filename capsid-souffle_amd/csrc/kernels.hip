@@ -100,6 +100,30 @@ __device__ __forceinline__ void bd_draws(uint32_t state, int b, double& ux, doub
 // replica's parameters are requested without waiting for it.
 __device__ __forceinline__ int rep_of(const Dev& d, int s) { return d.R == 1 ? 0 : d.sub_rep[s]; }
 
+// Row shift of a double across lanes (DPP row_shr:N within rows of 16 lanes; lanes without a
+// source get 0).
+template <int N>
+__device__ __forceinline__ double dpp_row_shr(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int l2 = __builtin_amdgcn_update_dpp(0, lo, 0x110 + N, 0xf, 0xf, true);
+  const int h2 = __builtin_amdgcn_update_dpp(0, hi, 0x110 + N, 0xf, 0xf, true);
+  return __hiloint2double(h2, l2);
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+// Sum over the 64 lanes of a wave (all lanes active), valid in every lane as a uniform value:
+// running sums within each row of 16 lanes by DPP shifts (the row total lands in its last lane),
+// then the four row totals in order. A fixed shape, so the result is deterministic.
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_row_shr<1>(v);
+  v += dpp_row_shr<2>(v);
+  v += dpp_row_shr<4>(v);
+  v += dpp_row_shr<8>(v);
+  return ((lane_value(v, 15) + lane_value(v, 31)) + lane_value(v, 47)) + lane_value(v, 63);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -553,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
     if (d2 > P.r_bound2 || (qi >= 0 && d2 > P.r_bound_q2)) S.deformed = 1;
   }
   // list sizes of this subunit for the statistics
-  const double wl = wave_sum(active ? (double)cnt_lj : 0.0), wq = wave_sum(active ? (double)cnt_q : 0.0);
+  const double wl = wave_total(active ? (double)cnt_lj : 0.0), wq = wave_total(active ? (double)cnt_q : 0.0);
   __syncthreads();
   double* red = reinterpret_cast<double*>(stage);
   if ((tid & 63) == 0) { red[2 * (tid >> 6)] = wl; red[2 * (tid >> 6) + 1] = wq; }
@@ -1177,7 +1201,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
     if (cnt_lj > d.lj_cap) atomicMax(&S.need_lj, cnt_lj);
   }
   // list sizes of this subunit for the statistics
-  const double wl = wave_sum(active ? (double)cnt_lj : 0.0), wq = wave_sum(active ? (double)cnt_q : 0.0);
+  const double wl = wave_total(active ? (double)cnt_lj : 0.0), wq = wave_total(active ? (double)cnt_q : 0.0);
   if ((tid & 63) == 0) { red[2 * (tid >> 6)] = wl; red[2 * (tid >> 6) + 1] = wq; }
   __syncthreads();
   if (tid == 0) {
@@ -1565,7 +1589,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   constexpr int kSums = MODE == MODE_MD ? 4 : 7;
 #pragma unroll
   for (int k = 0; k < kSums; k++) {
-    const double w = wave_sum(part[k]);
+    const double w = wave_total(part[k]);
     if ((tid & 63) == 0) red[tid >> 6][k] = w;
   }
   __syncthreads();
@@ -1626,7 +1650,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
 #pragma unroll
   for (int k = 0; k < kPartSlots; k++) {
     const bool used = k < 4 || (k < 7 ? brownian != 0 : lists);  // uniform over the workgroup
-    const double w = used ? wave_sum(acc[k]) : 0.0;
+    const double w = used ? wave_total(acc[k]) : 0.0;
     if ((tid & 63) == 0) wred[tid >> 6][k] = w;
   }
   __syncthreads();
