@@ -19,6 +19,9 @@
 //   - pair energies               src/energies.cpp:13-93
 //   - step loop body (NH and BD)  src/md.cpp:400-514, bead updates src/bead.h:72-149
 //   - oligomers (mass spectrum)   src/analysis.cpp:143-186
+//   - encapsulation big beads     placement src/initialize.cpp:566-680, forces
+//                                  src/forces.cpp:174-236, energies src/energies.cpp:96-136,
+//                                  their BD draws src/md.cpp:482-487
 // Templated on the real type R: R = long double mirrors the reference's VECTOR3D
 // (src/vector3d.h:22); R = double is the FP64 restatement the GPU path is compared with.
 // Quantities the reference keeps in `double` (edge length, face area, bead energies,
@@ -32,6 +35,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <random>
@@ -183,6 +187,11 @@ struct Sys : SysBase {
   std::uniform_real_distribution<> distr;
   bool list_overflow = false;
   long long list_entries = 0;
+  // encapsulation big beads (src/md.cpp:183-185): immobile; one diameter, charge and
+  // hard-core distance for all of them
+  std::vector<Vec> bpos;
+  double bsig = 0, bq = 0, bshc = 0;
+  std::vector<double> bce, bne;  // energies of each big bead
 
   Sys() : gen(69521u), distr(-0.5, 0.5) {}
 
@@ -529,6 +538,38 @@ struct Sys : SysBase {
     for (int i = 0; i < N; i++) tforce[i] = forvec[i];
   }
 
+  // src/forces.cpp:174-236: big bead i on subunit bead j, i outer and j inner. Only the
+  // big-bead terms go into the bead's eforce / ljforce there, and its total becomes
+  // tforce + ljforce + eforce.
+  void forces_bigbead() {
+    const int nbig = (int)bpos.size();
+    if (nbig == 0) return;
+    std::vector<Vec> ef(N), lf(N);
+    const double shc = bshc, shc6 = std::pow(shc, 6);
+    for (int i = 0; i < nbig; i++)
+      for (int j = 0; j < N; j++) {
+        const bool electrostatic = (q[j] != 0);
+        Vec rv = bpos[i] - pos[j];
+        min_image(rv);
+        const R r = rv.mag(), r2 = rv.mag2();
+        const double sig1 = bsig, sig2 = sigma[j], sigavg = (sig1 + sig2) / 2, delta = sigavg - shc;
+        const double r6 = (double)std::pow((R)(r - delta), 6);
+        const double ys = std::exp(kappa * sigavg) / ((1 + kappa * sig1 / 2) * (1 + kappa * sig2 / 2));
+        Vec eF, ljF;
+        if (electrostatic && r2 < (ecut_el * ecut_el))
+          eF = rv.scale((R)((bq * q[j] * lb * std::exp(-kappa * r) / (r2)) * (kappa + 1 / r) * ys));
+        ef[j] -= eF;
+        if (r2 < (delta + 1.12246205 * shc) * (delta + 1.12246205 * shc))
+          ljF = rv.scale((R)(48 * ((shc6 / r6) * ((shc6 / r6) - 0.5)) * (1 / r) * (1 / (r - delta))));
+        lf[j] -= ljF;
+      }
+    for (int j = 0; j < N; j++) tforce[j] = tforce[j] + lf[j] + ef[j];
+  }
+  void all_forces(bool rebuild, std::vector<Vec>* Fe_out = nullptr, std::vector<Vec>* Flj_out = nullptr) {
+    forces(rebuild, Fe_out, Flj_out);
+    forces_bigbead();
+  }
+
   // ---------------------------------------------------------------- energies
   long double particle_kinetic_energy() {  // src/energies.cpp:13-22
     for (int i = 0; i < N; i++) { R v = vel[i].mag(); ke[i] = (double)(0.5 * m[i] * v * v); }
@@ -579,6 +620,31 @@ struct Sys : SysBase {
       ne[i] = nei;
     }
   }
+  // src/energies.cpp:96-136: each big-bead / subunit-bead term is added to both beads'
+  // energies (the subunit bead's at full value)
+  void energies_bigbead() {
+    const int nbig = (int)bpos.size();
+    bce.assign(nbig, 0.0);
+    bne.assign(nbig, 0.0);
+    const double shc = bshc, shc6 = std::pow(shc, 6);
+    for (int i = 0; i < nbig; i++)
+      for (int j = 0; j < N; j++) {
+        const bool electrostatic = (q[j] != 0);
+        Vec rv = bpos[i] - pos[j];
+        min_image(rv);
+        const R r = rv.mag();
+        const double sig1 = bsig, sig2 = sigma[j], sigavg = (sig1 + sig2) / 2, delta = sigavg - shc;
+        const double r6 = (double)std::pow((R)(r - delta), 6);
+        const double ys = std::exp(kappa * sigavg) / ((1 + kappa * sig1 / 2) * (1 + kappa * sig2 / 2));
+        double cej = 0, nej = 0;
+        if (electrostatic && r < ecut_el) cej = (double)((bq * q[j] * lb * std::exp(-kappa * r) * ys) / (r));
+        bce[i] += cej;
+        ce[j] += cej;
+        if (r < delta + 1.12246205 * shc) nej = ((4 * (shc6 / r6) * ((shc6 / r6) - 1)) + 1);
+        bne[i] += nej;
+        ne[j] += nej;
+      }
+  }
   // src/md.cpp:553-623: returns ke, se, be, ne, ce, tpe, tke (totals)
   void energies(double* out) {
     for (size_t e = 0; e < eb0.size(); e++) update_length((int)e);  // dress_up
@@ -594,6 +660,7 @@ struct Sys : SysBase {
         if (etype[e] != 0) bending_energy(e, cli.kb);
     }
     lj_es_energies();
+    energies_bigbead();
     double S = 0, K = 0, L = 0, B = 0, Ce = 0, tp = 0, tk = 0;
     for (int i = 0; i < N; i++) { S += se[i]; K += ke[i]; L += ne[i]; B += be[i]; Ce += ce[i]; }
     for (auto& b : bath) {
@@ -620,7 +687,7 @@ struct Sys : SysBase {
   // initial force call and kinetic energy (src/md.cpp:241, 258)
   void start() {
     list_overflow = false;
-    forces(true);
+    all_forces(true);
     particle_ke = (double)particle_kinetic_energy();
   }
 
@@ -640,7 +707,7 @@ struct Sys : SysBase {
       }
     }
     bool rebuild = (a % cli.B_build == 0);
-    forces(rebuild);
+    all_forces(rebuild);
     if (!cli.brownian) {
       for (int i = 0; i < N; i++) therm_update_velocity(i, dt, expfac_real);
       particle_ke = (double)particle_kinetic_energy();
@@ -657,6 +724,8 @@ struct Sys : SysBase {
       for (int i = 0; i < N; i++) avg = avg + fran[i];
       avg = avg.scale((R)(1.0 / N));
       for (int i = 0; i < N; i++) fran[i] = fran[i] - avg;
+      // the big beads' noise is drawn next and never used (src/md.cpp:482-491)
+      for (size_t k = 0; k < 3 * bpos.size(); k++) distr(gen);
       for (int i = 0; i < N; i++) {
         tforce[i] = tforce[i] + fran[i] + fdrag[i];
         update_velocity(i, dt);
@@ -796,7 +865,7 @@ int orc_forces(void* hv, int rebuild, double* F, double* Fs, double* Fb, double*
     typedef typename std::remove_reference<decltype(s)>::type S;
     std::vector<typename S::Vec> keep = s.tforce, fe(s.N), flj(s.N);
     s.list_overflow = false;
-    s.forces(rebuild != 0, &fe, &flj);
+    s.all_forces(rebuild != 0, &fe, &flj);
     for (int i = 0; i < s.N; i++) {
       auto put = [&](double* o, const typename S::Vec& v) {
         if (o) { o[3 * i] = (double)v.x; o[3 * i + 1] = (double)v.y; o[3 * i + 2] = (double)v.z; }
@@ -890,6 +959,94 @@ int orc_hinges(void* hv, int* out, int cap) {
 // (either may be NULL). Returns the number of oligomers.
 int orc_oligomers(void* hv, double contact, int* label, int* ms) {
   return dispatch((OrcHandle*)hv, [&](auto& s) { return s.oligomers(contact, label, ms); });
+}
+
+// Encapsulation big beads: n immobile beads at xyz[3n] of diameter sigma, charge q (e) and
+// hard-core distance shc; n = 0 removes them. Positions are taken through the minimum
+// image only, as the reference does. Takes effect at the next force call.
+int orc_set_big_beads(void* hv, int n, const double* xyz, double sigma, double q, double shc) {
+  return dispatch((OrcHandle*)hv, [&](auto& s) {
+    typedef typename std::remove_reference<decltype(s)>::type S;
+    s.bpos.clear();
+    for (int i = 0; i < n; i++) s.bpos.push_back(typename S::Vec(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]));
+    s.bsig = sigma; s.bq = q; s.bshc = shc;
+    return 0;
+  });
+}
+
+// Per-bead diameter and charge (nb = N), either may be NULL.
+int orc_bead_props(void* hv, double* sigma, double* q) {
+  return dispatch((OrcHandle*)hv, [&](auto& s) {
+    for (int i = 0; i < s.N; i++) {
+      if (sigma) sigma[i] = s.sigma[i];
+      if (q) q[i] = s.q[i];
+    }
+    return s.N;
+  });
+}
+
+// Energies of the big beads from the last orc_energies call: out = {sum of ce, sum of ne}.
+int orc_big_energies(void* hv, double* out) {
+  return dispatch((OrcHandle*)hv, [&](auto& s) {
+    double c = 0, l = 0;
+    for (size_t i = 0; i < s.bce.size(); i++) { c += s.bce[i]; l += s.bne[i]; }
+    out[0] = c; out[1] = l;
+    return 0;
+  });
+}
+
+// generate_big_beads (src/initialize.cpp:566-680) without a restart file: srand(0), then per
+// big bead random positions (three rand() calls, x, y, z) until one keeps a distance of at
+// least (sigma_j + sigma) / 2 from every existing bead j and sigma from every big bead
+// placed before (minimum image by rounding). Existing beads: pos[3 nb], sig[nb]; cubic
+// box. The reference's positions and box are long double; so is the arithmetic here.
+// Returns 0, or -1 (as the reference's exit) when a bead needs max_attempts tries.
+int orc_place_big_beads(const double* pos, const double* sig, int nb, double box, int n, double sigma,
+                        unsigned max_attempts, double* out) {
+  typedef long double LD;
+  const LD bx = box;
+  const double R_squared = sigma * sigma;
+  std::srand(0);
+  int placed_n = 0;
+  for (int i = 0; i < n; i++) {
+    bool placed = false;
+    unsigned attempts = 0;
+    while (!placed && attempts < max_attempts) {
+      const double x = (static_cast<double>(std::rand()) / RAND_MAX - 0.5) * bx;
+      const double y = (static_cast<double>(std::rand()) / RAND_MAX - 0.5) * bx;
+      const double z = (static_cast<double>(std::rand()) / RAND_MAX - 0.5) * bx;
+      bool collision = false;
+      for (int j = 0; j < nb && !collision; j++) {
+        double dx = x - (LD)pos[3 * j];
+        dx -= bx * std::floor(dx / bx + 0.5);
+        double dy = y - (LD)pos[3 * j + 1];
+        dy -= bx * std::floor(dy / bx + 0.5);
+        double dz = z - (LD)pos[3 * j + 2];
+        dz -= bx * std::floor(dz / bx + 0.5);
+        const double dist_sq = dx * dx + dy * dy + dz * dz;
+        const double min_dist = sig[j] / 2 + sigma / 2;
+        if (dist_sq < min_dist * min_dist) collision = true;
+      }
+      for (int k = 0; k < placed_n && !collision; k++) {
+        double dx = x - (LD)out[3 * k];
+        dx -= bx * std::floor(dx / bx + 0.5);
+        double dy = y - (LD)out[3 * k + 1];
+        dy -= bx * std::floor(dy / bx + 0.5);
+        double dz = z - (LD)out[3 * k + 2];
+        dz -= bx * std::floor(dz / bx + 0.5);
+        const double dist_sq = dx * dx + dy * dy + dz * dz;
+        if (dist_sq < R_squared) collision = true;
+      }
+      if (!collision) {
+        out[3 * placed_n] = x; out[3 * placed_n + 1] = y; out[3 * placed_n + 2] = z;
+        placed_n++;
+        placed = true;
+      }
+      attempts++;
+    }
+    if (attempts >= max_attempts) return -1;
+  }
+  return 0;
 }
 
 // The BD noise stream of the reference: n uniform(-0.5, 0.5) draws of

@@ -38,6 +38,12 @@ def load():
         lib.orc_bd_uniforms.argtypes = [ctypes.c_longlong, ctypes.c_int, _D]
         lib.orc_oligomers.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
                                       ctypes.POINTER(ctypes.c_int)]
+        lib.orc_set_big_beads.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double]
+        lib.orc_big_energies.argtypes = [ctypes.c_void_p, _D]
+        lib.orc_bead_props.argtypes = [ctypes.c_void_p, _D, _D]
+        lib.orc_place_big_beads.argtypes = [_D, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_uint, _D]
         lib.orc_last_error.restype = ctypes.c_char_p
         _lib = lib
     return _lib
@@ -128,6 +134,24 @@ class Oracle:
         n = load().orc_oligomers(self.ptr, contact, lab.ctypes.data_as(ip), ms.ctypes.data_as(ip))
         return lab, ms, n
 
+    def bead_props(self):
+        """(sigma[N], q[N]) of the subunit beads."""
+        sig = np.zeros(self.N)
+        q = np.zeros(self.N)
+        load().orc_bead_props(self.ptr, _dp(sig), _dp(q))
+        return sig, q
+
+    def set_big_beads(self, xyz, sigma, q, shc):
+        """Encapsulation big beads (immobile) at xyz[n][3]; effective from the next force call."""
+        xyz = np.ascontiguousarray(np.reshape(xyz, (-1, 3)), dtype=np.float64)
+        load().orc_set_big_beads(self.ptr, xyz.shape[0], _dp(xyz), sigma, q, shc)
+
+    def big_energies(self):
+        """(sum of ce, sum of ne) over the big beads from the last energies() call."""
+        out = np.zeros(2)
+        load().orc_big_energies(self.ptr, _dp(out))
+        return out
+
     def close(self):
         if getattr(self, "ptr", None):
             load().orc_destroy(self.ptr)
@@ -143,6 +167,16 @@ def bd_uniforms(skip, n):
     out = np.zeros(n)
     load().orc_bd_uniforms(skip, n, _dp(out))
     return out
+
+
+def place_big_beads(pos, sig, box, n, sigma, max_attempts=1000000):
+    """The reference's big-bead placement (srand(0), rand() positions, overlap rejection);
+    None when a bead needs max_attempts tries."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    sig = np.ascontiguousarray(sig, dtype=np.float64)
+    out = np.zeros((n, 3))
+    rc = load().orc_place_big_beads(_dp(pos), _dp(sig), sig.shape[0], box, n, sigma, max_attempts, _dp(out))
+    return None if rc else out
 
 
 def rel_norm(a, b):
