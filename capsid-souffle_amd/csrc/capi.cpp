@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -143,6 +144,8 @@ struct sf_handle {
   // far field of the uncut Yukawa (split mode only): the model when one is attached (energies
   // always use the exact far field)
   int launch_far(int energy) {
+    if (d.nbig)
+      if (int rc = launch_big(d, st)) return rc;
     if (!d.split) return SF_OK;
     if (far_drop && !energy) return launch_fill(d.qfar, 3 * (size_t)d.Nq, 0.0, st);
     if (far_model && !energy) {
@@ -370,6 +373,14 @@ struct sf_handle {
   int g_cap[2] = {0, 0};
   int g_tile = 0;
   bool g_far = false, g_drop = false;
+  int g_nbig = 0;
+  // encapsulation big beads: buffers (allocated on first use) and whether the set changed
+  // since the last start
+  int nbig_cap = 0;
+  double4* big_pos = nullptr;
+  double4* big_tab = nullptr;
+  double4* big_f = nullptr;
+  bool big_dirty = false;
   const void* g_cell[4] = {nullptr, nullptr, nullptr, nullptr};
   int g_ccap[2] = {0, 0};
   std::vector<hipEvent_t> events;
@@ -401,7 +412,7 @@ struct sf_handle {
     // the captured kernel arguments hold the list buffers and their capacities; a regrown
     // buffer may come back at the same address, so the capacities are part of the key
     if (!gexec || g_n != n || g_tm != timing || g_key[0] != d.lj_list || g_key[1] != d.q_list ||
-        g_cap[0] != d.lj_cap || g_cap[1] != d.q_cap || g_tile != d.tile || g_far != far_model ||
+        g_cap[0] != d.lj_cap || g_cap[1] != d.q_cap || g_tile != d.tile || g_far != far_model || g_nbig != d.nbig ||
         g_drop != far_drop || g_cell[0] != d.cell_lj || g_cell[1] != d.cnt_lj || g_cell[2] != d.cell_q ||
         g_cell[3] != d.cnt_q || g_ccap[0] != d.ccap_lj || g_ccap[1] != d.ccap_q) {
       drop_graph();
@@ -415,7 +426,7 @@ struct sf_handle {
       (void)hipGraphDestroy(g);
       if (ei != hipSuccess) { gexec = nullptr; set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return SF_E_HIP; }
       g_n = n; g_tm = timing; g_key[0] = d.lj_list; g_key[1] = d.q_list;
-      g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap; g_tile = d.tile; g_far = far_model; g_drop = far_drop;
+      g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap; g_tile = d.tile; g_far = far_model; g_drop = far_drop; g_nbig = d.nbig;
       g_cell[0] = d.cell_lj; g_cell[1] = d.cnt_lj; g_cell[2] = d.cell_q; g_cell[3] = d.cnt_q;
       g_ccap[0] = d.ccap_lj; g_ccap[1] = d.ccap_q;
     }
@@ -644,6 +655,7 @@ int sf_set_state(sf_handle* h, const double* pos, const double* vel) {
   int rc = h->start_sequence();
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
+  h->big_dirty = false;
   return SF_OK;
 }
 
@@ -733,6 +745,7 @@ int sf_energies(sf_handle* h, double* totals, double* per_bead) {
 
 int sf_step(sf_handle* h, int n_steps) {
   if (!h || n_steps < 0) { set_error("bad arguments to sf_step"); return SF_E_ARG; }
+  if (h->big_dirty) { set_error("big beads changed since the last sf_set_state: restart from the new set first"); return SF_E_ARG; }
   double ti = 0, trb = 0, tf = 0, tr = 0, nt = 0, tfar = 0;
   int done = 0, rc = SF_OK;
   int retries = 0;
@@ -804,8 +817,9 @@ int sf_get_stats(sf_handle* h, sf_stats* o) {
 int sf_bd_uniforms(sf_handle* h, int replica, int64_t step, double* out) {
   if (!h || replica < 0 || replica >= h->R || step < 0 || !out) { set_error("bad arguments"); return SF_E_ARG; }
   const RepParams& P = h->rp[replica];
-  uint64_t e = 6ull * (uint64_t)P.nb;
-  // state before iteration `step`: seed * 16807^(6 nb step) mod (2^31-1)
+  // two engine calls per draw; the big beads' draws follow the subunit beads' every step
+  uint64_t e = 6ull * ((uint64_t)P.nb + (uint64_t)h->d.nbig);
+  // state before iteration `step`: seed * 16807^(6 (nb + n_big) step) mod (2^31-1)
   uint64_t st = P.seed;
   uint64_t mul = 1, base = lcg_pow(e);
   for (uint64_t k = (uint64_t)step; k; k >>= 1) {
@@ -988,6 +1002,109 @@ int sf_get_far_stats(sf_handle* h, sf_far_stats* o) {
   if (!h || !o) { set_error("null argument"); return SF_E_ARG; }
   o->audits = h->far_audits; o->last_err = h->far_last; o->max_err = h->far_max;
   o->using_model = h->far_model ? 1 : 0; o->fell_back = h->far_fell_back ? 1 : 0;
+  return SF_OK;
+}
+
+int sf_set_big_beads(sf_handle* h, int n, const double* xyz, double sigma, double charge, double shc) {
+  if (!h || n < 0 || (n > 0 && !xyz)) { set_error("bad arguments to sf_set_big_beads"); return SF_E_ARG; }
+  if (n > 0 && h->R != 1) { set_error("big beads need a single-replica handle"); return SF_E_ARG; }
+  if (n > 0 && !(sigma > 0)) { set_error("big-bead diameter must be positive"); return SF_E_ARG; }
+  HIPCHK(hipStreamSynchronize(h->st));  // no launch may read the tables being replaced
+  h->drop_graph();
+  RepParams& P = h->rp[0];
+  if (n > 0) {
+    const int np = h->t.np;
+    if (n > h->nbig_cap) {
+      if (h->alloc(&h->big_pos, (size_t)n)) return SF_E_HIP;
+      h->nbig_cap = n;
+    }
+    if (!h->big_f && (h->alloc(&h->big_f, 2 * (size_t)h->Nb) || h->alloc(&h->big_tab, 2 * (size_t)np)))
+      return SF_E_HIP;
+    std::vector<double4> bp(n), tab(2 * (size_t)np);
+    for (int i = 0; i < n; i++) bp[i] = make_double4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.0);
+    // per template bead, as src/forces.cpp:209-219 forms them
+    const double shc6 = std::pow(shc, 6);
+    for (int l = 0; l < np; l++) {
+      const double sig1 = sigma, sig2 = h->t.sigma[l], sigavg = (sig1 + sig2) / 2, delta = sigavg - shc;
+      const double ys = std::exp(P.kappa * sigavg) / ((1 + P.kappa * sig1 / 2) * (1 + P.kappa * sig2 / 2));
+      tab[2 * l] = make_double4(delta, delta + 1.12246205 * shc,
+                                (delta + 1.12246205 * shc) * (delta + 1.12246205 * shc), ys);
+      tab[2 * l + 1] = make_double4(charge * h->t.q[l] * P.lb, h->t.q[l] != 0 ? 1.0 : 0.0,
+                                    P.ecut_el * P.ecut_el, shc6);
+    }
+    HIPCHK(hipMemcpy(h->big_pos, bp.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->big_tab, tab.data(), tab.size() * sizeof(double4), hipMemcpyHostToDevice));
+  }
+  h->d.nbig = n;
+  h->d.big_pos = h->big_pos;
+  h->d.big_tab = h->big_tab;
+  h->d.big_f = h->big_f;
+  // BD: each step draws the big beads' noise after the subunit beads' (src/md.cpp:482-487)
+  P.mult6n = lcg_pow(6ull * ((unsigned long long)P.nb + (unsigned long long)n));
+  HIPCHK(hipMemcpy(const_cast<RepParams*>(h->d.rp), &P, sizeof(RepParams), hipMemcpyHostToDevice));
+  h->big_dirty = true;
+  return SF_OK;
+}
+
+int sf_big_energies(sf_handle* h, double* out) {
+  if (!h || !out) { set_error("null argument"); return SF_E_ARG; }
+  out[0] = out[1] = 0;
+  if (!h->d.nbig) return SF_OK;
+  if (int rc = launch_big(h->d, h->st)) return rc;
+  std::vector<double4> f(2 * (size_t)h->Nb);
+  HIPCHK(hipMemcpyAsync(f.data(), h->big_f, f.size() * sizeof(double4), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  double ce = 0, ne = 0;
+  for (size_t b = 0; b < (size_t)h->Nb; b++) { ne += f[2 * b].w; ce += f[2 * b + 1].w; }
+  out[0] = ce; out[1] = ne;
+  return SF_OK;
+}
+
+int sf_big_beads_place(const sf_template* t, int n_subunits, const double* pos, double box, int n, double sigma,
+                       unsigned max_attempts, double* out) {
+  if (!t || n_subunits < 0 || (n_subunits > 0 && !pos) || n < 0 || (n > 0 && !out) || !(box > 0)) {
+    set_error("bad arguments to sf_big_beads_place");
+    return SF_E_ARG;
+  }
+  // src/initialize.cpp:566-680 without a restart file; positions and box are long double there
+  typedef long double LD;
+  const LD bx = box;
+  const int np = t->np;
+  const size_t nb = (size_t)n_subunits * np;
+  const double r2_big = sigma * sigma;
+  auto wrap = [&](double v) { return v - bx * std::floor(v / bx + 0.5); };
+  std::srand(0);
+  for (int i = 0; i < n; i++) {
+    bool placed = false;
+    unsigned attempts = 0;
+    while (!placed && attempts < max_attempts) {
+      double c[3];
+      for (int k = 0; k < 3; k++) c[k] = (double)((static_cast<double>(std::rand()) / RAND_MAX - 0.5) * bx);
+      bool hit = false;
+      for (size_t j = 0; j < nb && !hit; j++) {
+        double dd[3];
+        for (int k = 0; k < 3; k++) dd[k] = wrap(c[k] - (LD)pos[3 * j + k]);
+        const double dist_sq = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+        const double min_dist = t->sigma[j % np] / 2 + sigma / 2;
+        hit = dist_sq < min_dist * min_dist;
+      }
+      for (int k = 0; k < i && !hit; k++) {
+        double dd[3];
+        for (int m = 0; m < 3; m++) dd[m] = wrap(c[m] - (LD)out[3 * k + m]);
+        hit = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] < r2_big;
+      }
+      if (!hit) {
+        for (int k = 0; k < 3; k++) out[3 * i + k] = c[k];
+        placed = true;
+      }
+      attempts++;
+    }
+    if (attempts >= max_attempts) {
+      set_error("failed to place big bead " + std::to_string(i + 1) + " after " + std::to_string(max_attempts) +
+                " attempts");
+      return SF_E_ARG;
+    }
+  }
   return SF_OK;
 }
 

@@ -1548,6 +1548,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     }
     const double elx = exd + (double)lx, ely = eyd + (double)ly, elz = ezd + (double)lz;
     double Fx = sx + bx + elx, Fy = sy + by + ely, Fz = sz + bz + elz;
+    if (d.nbig) {  // src/forces.cpp:234-235: tforce + ljforce + eforce of the big-bead terms
+      const double4 bl = d.big_f[2 * (size_t)gi], be = d.big_f[2 * (size_t)gi + 1];
+      Fx = (Fx + bl.x) + be.x; Fy = (Fy + bl.y) + be.y; Fz = (Fz + bl.z) + be.z;
+    }
     if (MODE == MODE_FORCES) {
       if (d.out_f) { d.out_f[3 * (size_t)gi] = Fx; d.out_f[3 * (size_t)gi + 1] = Fy; d.out_f[3 * (size_t)gi + 2] = Fz; }
       if (d.out_s) { d.out_s[3 * (size_t)gi] = sx; d.out_s[3 * (size_t)gi + 1] = sy; d.out_s[3 * (size_t)gi + 2] = sz; }
@@ -1584,7 +1588,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
       part[4] = amp * ux; part[5] = amp * uy; part[6] = amp * uz;
     }
   }
-  // per-subunit sums, fixed shape: wave shuffles then the two waves in order (slots 7-8 are
+  // per-subunit sums, fixed shape: DPP wave sums then the two waves in order (slots 7-8 are
   // the list sizes written by k_rebuild; the BD noise slots 4-6 are not used by NH MD)
   constexpr int kSums = MODE == MODE_MD ? 4 : 7;
 #pragma unroll
@@ -1904,6 +1908,10 @@ __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
       }
       if (P.split) e[3] += d.efar[gq];
     }
+    if (d.nbig) {  // src/energies.cpp:129-134: the big-bead terms at full value
+      e[2] += d.big_f[2 * (size_t)gi].w;
+      e[3] += d.big_f[2 * (size_t)gi + 1].w;
+    }
     const double m = d.topo.mass[tid];
     const double vx = d.vx[gi], vy = d.vy[gi], vz = d.vz[gi];
     e[4] = 0.5 * m * (vx * vx + vy * vy + vz * vz);
@@ -1917,6 +1925,46 @@ __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
   }
   __syncthreads();
   if (tid < 5) d.epart[(size_t)s * 5 + tid] = red[0][tid] + red[1][tid];
+}
+
+// Encapsulation big beads on the subunit beads (src/forces.cpp:174-236) and their energies
+// (src/energies.cpp:96-136): one thread per subunit bead, big beads in order. The immobile
+// big beads exert a Yukawa force on charged beads within ecut_el and a WCA force shifted by
+// delta = (sigma_big + sigma_j) / 2 - shc; each term is subtracted from the bead's running
+// sums as there (the reference's vector is big bead minus subunit bead).
+__global__ __launch_bounds__(256) void k_big(Dev d) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= d.Nb) return;
+  const double4 p = d.pos[b];
+  const int l = tag_loc(p.w);
+  const RepParams& P = d.rp[0];
+  const double box = P.box, hbox = P.hbox, kappa = P.kappa, ecut_el = P.ecut_el;
+  const double4 t0 = d.big_tab[2 * l], t1 = d.big_tab[2 * l + 1];
+  const double delta = t0.x, wca_cut = t0.y, wca_cut2 = t0.z, ys = t0.w;
+  const double qqlb = t1.x, ecut_el2 = t1.z, shc6 = t1.w;
+  const bool charged = t1.y != 0;
+  double lx = 0, ly = 0, lz = 0, ex = 0, ey = 0, ez = 0, ne = 0, ce = 0;
+  for (int i = 0; i < d.nbig; i++) {
+    const double4 c = d.big_pos[i];
+    const double rx = min_img(c.x - p.x, box, hbox), ry = min_img(c.y - p.y, box, hbox),
+                 rz = min_img(c.z - p.z, box, hbox);
+    const double r2 = rx * rx + ry * ry + rz * rz, r = sqrt(r2);
+    const double u = r - delta, u2 = u * u, r6 = u2 * u2 * u2;
+    const double x6 = shc6 / r6;
+    const double ek = charged && (r2 < ecut_el2 || r < ecut_el) ? exp(-kappa * r) : 0.0;
+    if (charged && r2 < ecut_el2) {
+      const double sc = ((qqlb * ek) / r2) * (kappa + 1 / r) * ys;
+      ex -= rx * sc; ey -= ry * sc; ez -= rz * sc;
+    }
+    if (r2 < wca_cut2) {
+      const double sc = 48 * (x6 * (x6 - 0.5)) * (1 / r) * (1 / u);
+      lx -= rx * sc; ly -= ry * sc; lz -= rz * sc;
+    }
+    if (charged && r < ecut_el) ce += (qqlb * ek * ys) / r;
+    if (r < wca_cut) ne += (4 * x6 * (x6 - 1)) + 1;
+  }
+  d.big_f[2 * (size_t)b] = make_double4(lx, ly, lz, ne);
+  d.big_f[2 * (size_t)b + 1] = make_double4(ex, ey, ez, ce);
 }
 
 __global__ void k_bd_uniforms(Dev d, uint32_t state, int nb, double* out) {
@@ -2023,6 +2071,11 @@ int launch_apply_com(const Dev& d, hipStream_t st) {
 int launch_energy(const Dev& d, hipStream_t st) {
   size_t sm = (size_t)d.topo.np * sizeof(double4) + (size_t)d.topo.nh * sizeof(double);
   hipLaunchKernelGGL(k_energy, dim3(d.Ns), dim3(kBlock), sm, st, d);
+  return check_launch();
+}
+
+int launch_big(const Dev& d, hipStream_t st) {
+  hipLaunchKernelGGL(k_big, dim3((d.Nb + 255) / 256), dim3(256), 0, st, d);
   return check_launch();
 }
 

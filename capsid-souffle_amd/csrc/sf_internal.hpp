@@ -182,7 +182,15 @@ struct Dev {
   int lj_cap, q_cap;
   int Nb, Ns, Nq, R;
   int split;           // 1: every replica uses the uncut near/far Yukawa split
-  int tile;            // list entries k_force stages in LDS per gather round (128, 256, 512)
+  int tile;            // list entries k_force stages in LDS per gather round (128, 256, 384, 512)
+  // encapsulation big beads (single-replica handles): immobile positions, per-template-bead
+  // constants [2 np] = (delta, wca_cut, wca_cut^2, yukawa_scale), (q_big q lb, charged,
+  // ecut_el^2, shc^6), and the big-bead terms of each subunit bead [2 Nb] = (ljforce, ne),
+  // (eforce, ce)
+  int nbig;
+  const double4* big_pos;
+  const double4* big_tab;
+  double4* big_f;
 };
 
 enum { MODE_FORCES = 0, MODE_MD = 1, MODE_BD = 2 };
@@ -208,6 +216,7 @@ int launch_far_exact(const Dev& d, int energy, hipStream_t st);
 int launch_fill(double* p, size_t n, double v, hipStream_t st);  // a kernel, so graphs may hold it
 int launch_apply_com(const Dev& d, hipStream_t st);
 int launch_energy(const Dev& d, hipStream_t st);
+int launch_big(const Dev& d, hipStream_t st);
 int launch_bd_uniforms_state(uint32_t state, int nb, double* out_dev, const Dev& d, hipStream_t st);
 int launch_oligomers(const Dev& d, double contact, OligoBuf& o, hipStream_t st);  // analysis.hip
 

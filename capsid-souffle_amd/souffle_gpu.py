@@ -111,6 +111,10 @@ SIGNATURES = {
     "sf_far_drop": (ctypes.c_int, [_P, ctypes.c_int]),
     "sf_get_far_stats": (ctypes.c_int, [_P, ctypes.POINTER(SfFarStats)]),
     "sf_oligomers": (ctypes.c_int, [_P, ctypes.c_double, _I, _I]),
+    "sf_big_beads_place": (ctypes.c_int, [_P, ctypes.c_int, _D, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_uint, _D]),
+    "sf_set_big_beads": (ctypes.c_int, [_P, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "sf_big_energies": (ctypes.c_int, [_P, _D]),
     "sf_last_error": (ctypes.c_char_p, []),
 }
 
@@ -322,6 +326,19 @@ class System:
             s0 += n
         return out
 
+    def set_big_beads(self, xyz, sigma=6.0, charge=-78.0, shc=3.5):
+        """Immobile encapsulation big beads at xyz[n][3] (defaults: the reference's -a 3, -G -78,
+        -H 3.5); applies from the next set_state(). An empty array removes them."""
+        xyz = np.ascontiguousarray(np.reshape(xyz, (-1, 3)), dtype=np.float64)
+        _err(_lib.sf_set_big_beads(self.ptr, xyz.shape[0], _dp(xyz), float(sigma), float(charge), float(shc)),
+             "sf_set_big_beads")
+
+    def big_energies(self):
+        """(sum of ce, sum of ne) of the big beads at the current positions."""
+        out = np.zeros(2)
+        _err(_lib.sf_big_energies(self.ptr, _dp(out)), "sf_big_energies")
+        return out
+
     def close(self):
         if getattr(self, "ptr", None):
             _lib.sf_destroy(self.ptr)
@@ -329,6 +346,18 @@ class System:
 
     def __del__(self):
         self.close()
+
+
+def place_big_beads(template, pos, box, n, sigma=6.0, max_attempts=1000000):
+    """The reference's big-bead placement (srand(0), rand() positions, overlap rejection) among
+    the subunit beads pos[N][3]; returns [n][3]."""
+    load()
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    ns = pos.shape[0] // template.n_beads
+    out = np.zeros((n, 3))
+    _err(_lib.sf_big_beads_place(template.ptr, ns, _dp(pos), float(box), int(n), float(sigma), int(max_attempts),
+                                 _dp(out)), "sf_big_beads_place")
+    return out
 
 
 def _fp(a):

@@ -305,6 +305,32 @@ int sf_get_far_stats(sf_handle* h, sf_far_stats* out);
  * Returns the total number of oligomers over all replicas, or a negative error code. */
 int sf_oligomers(sf_handle* h, double contact, int* label, int* mass_spectrum);
 
+/* ---- encapsulation big beads ------------------------------------------------------------ */
+
+/* Places n big beads of diameter sigma among the subunit beads pos[n_subunits * n_beads][3] of
+ * a cubic box, as generate_big_beads (src/initialize.cpp:566-680) does without a restart file:
+ * srand(0), then for each big bead rand() positions (x, y, z) until one lies at least
+ * (sigma_j + sigma) / 2 from every subunit bead j and sigma from every big bead placed before
+ * (minimum image by rounding). out[n][3]. Host code only (the C library's rand(), so the same
+ * stream as the reference on the same C library). Fails with SF_E_ARG when a bead needs
+ * max_attempts tries (the reference exits there). */
+int sf_big_beads_place(const sf_template* t, int n_subunits, const double* pos, double box, int n,
+                       double sigma, unsigned max_attempts, double* out);
+
+/* Immobile big beads (replaces `-p` encapsulation, src/md.cpp:183-185, 242, 444): n beads at
+ * xyz[n][3] of diameter sigma, charge `charge` (e) and hard-core distance shc, acting on every
+ * subunit bead with the forces of forceCalculation_bigbead (src/forces.cpp:174-236) and the
+ * energies of update_LJ_ES_energies_bigbeads (src/energies.cpp:96-136, added to the beads' ne and
+ * ce in sf_energies). In BD their noise is drawn after the subunit beads' every step and not
+ * used (src/md.cpp:482-491), so the noise stream matches the reference's. Single-replica
+ * handles only; n = 0 removes them. The new set applies from the next sf_set_state (sf_step
+ * refuses to run before it); sf_forces and sf_energies use it at once. */
+int sf_set_big_beads(sf_handle* h, int n, const double* xyz, double sigma, double charge, double shc);
+
+/* Energies of the big beads at the current positions: out[2] = {sum of ce, sum of ne}
+ * (the reference's big_cenergy and big_ljenergy, src/md.cpp:606-613). */
+int sf_big_energies(sf_handle* h, double* out);
+
 const char* sf_last_error(void);
 
 #ifdef __cplusplus

@@ -81,3 +81,21 @@ def test_make_state():
     assert np.sqrt((d * d).sum(-1)).min() >= 1.0 - 1e-12
     pos2, vel2 = t.make_state(200, p.box, 1.3, 17)
     assert np.array_equal(pos, pos2) and np.array_equal(vel, vel2)
+
+
+def test_big_bead_placement_matches_oracle():
+    """sf_big_beads_place against the oracle's generate_big_beads restatement
+    (src/initialize.cpp:566-680): the same rand() stream and long double arithmetic, so the
+    positions agree bit for bit; too few attempts is an error, as the reference's exit(1)."""
+    from oracle_lib import place_big_beads as orc_place
+    from souffle_states import make_state, read_template_xyz
+    t = sg.Template(TEMPLATE)
+    o = Oracle(20)
+    pos, _ = make_state(read_template_xyz(TEMPLATE)[0], 20, o.box, 3, spacing=8.6, perturb=0.05)
+    sig, _ = o.bead_props()
+    for n in (1, 4):
+        got = sg.place_big_beads(t, pos, o.box, n)
+        ref = orc_place(pos, sig, o.box, n, 6.0)
+        assert np.array_equal(got, ref)
+    with pytest.raises(sg.SfError):
+        sg.place_big_beads(t, pos, o.box, 3, max_attempts=1)
