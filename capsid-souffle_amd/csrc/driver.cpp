@@ -4,6 +4,10 @@
 // the outfiles/ writers: energy.out (src/initialize.cpp:12-19, src/md.cpp:339-346, 616-623),
 // model.parameters.out (src/md.cpp:112-180, 234), restart_<a>.out (src/md.cpp:526-536),
 // ovito.lammpstrj (src/md.cpp:647-669) and the progress bar (src/functions.cpp:24-30).
+// Encapsulation (-p, -a, -G, -m, -H): immobile big beads placed as src/initialize.cpp:566-680
+// (or read from the newest outfiles/rb_*.out on a restart), capenergy.out rows
+// (src/md.cpp:347-352, 625-630), rb_<a>.out beside each restart file (src/md.cpp:537-545) and
+// the big beads appended to each movie frame (src/md.cpp:649-676).
 // The step loop runs on the GPU in blocks between output steps.
 //
 // Initial state without a restart file: the reference's cubic lattice of identically oriented
@@ -118,14 +122,14 @@ int parse(int argc, char** argv, Opts& o) {
   return 0;
 }
 
-// The newest restart file in outfiles/ (names containing "restart", ordered by length then
-// lexicographically, src/functions.cpp:262-307).
-std::string newest_restart() {
+// The newest file in outfiles/ whose name contains `key` ("restart", or "rb" for the big
+// beads), ordered by length then lexicographically (src/functions.cpp:262-307).
+std::string newest_restart(const char* key = "restart") {
   std::vector<std::string> names;
   if (DIR* d = opendir("outfiles")) {
     while (dirent* e = readdir(d)) {
       std::string n = e->d_name;
-      if (n.find("restart") != std::string::npos) names.push_back(n);
+      if (n.find(key) != std::string::npos) names.push_back(n);
     }
     closedir(d);
   }
@@ -181,11 +185,11 @@ int main(int argc, char** argv) {
   Opts o;
   int pr = parse(argc, argv, o);
   if (pr != 0) return pr < 0 ? 1 : 0;
-  if (o.p) { std::cerr << "encapsulation big beads (-p) are not part of this engine\n"; return 1; }
   if (o.X) { std::cerr << "cluster start (-X) needs infiles/xyz_T4, which this engine does not read\n"; return 1; }
   mkdir("outfiles", 0755);
   const bool brownian = o.response == 'b';
   std::ofstream traj("outfiles/energy.out", std::ios_base::app);
+  std::ofstream encapdata("outfiles/capenergy.out", std::ios::out);
   std::ofstream ofile("outfiles/ovito.lammpstrj", std::ios_base::app);
   std::ofstream sysdata("outfiles/model.parameters.out", std::ios::out);
   traj << "time" << std::setw(15) << "kinetic" << std::setw(15) << "stretching" << std::setw(15) << "bending"
@@ -317,6 +321,34 @@ int main(int argc, char** argv) {
 
   sf_handle* h = sf_create(tmpl, 1, &ns, &prm);
   if (!h) return fail("sf_create");
+  // encapsulation big beads (src/md.cpp:183-185, src/initialize.cpp:566-680): immobile, of
+  // diameter 2a, charge G, mass 100 and type 3; read from the newest rb file on a restart,
+  // placed among the subunit beads otherwise
+  const int nbig = o.p ? o.m : 0;
+  std::vector<double> big_pos(3 * (size_t)nbig), big_vel(3 * (size_t)nbig, 0.0);
+  const double big_sigma = 2 * o.a, big_mass = 100.0;
+  const int big_type = 3;
+  double big_ke = 0;
+  if (nbig > 0) {
+    if (restart) {
+      const std::string rb = newest_restart("rb");
+      std::ifstream bin(("outfiles/" + rb).c_str());
+      for (int i = 0; i < nbig; i++)
+        bin >> big_vel[3 * i] >> big_vel[3 * i + 1] >> big_vel[3 * i + 2] >> big_pos[3 * i] >> big_pos[3 * i + 1] >>
+            big_pos[3 * i + 2];
+      if (rb.empty() || !bin) { std::cerr << "big-bead restart file (outfiles/rb_*.out) missing or short\n"; return 1; }
+    } else if (sf_big_beads_place(tmpl, ns, pos.data(), prm.box, nbig, big_sigma, 1000000u, big_pos.data())) {
+      std::cerr << "Error: " << sf_last_error() << std::endl;
+      return 1;
+    }
+    for (int i = 0; i < nbig; i++) {  // BEAD::update_kinetic_energy (src/bead.h:129-131)
+      const long double v = std::sqrt((long double)big_vel[3 * i] * big_vel[3 * i] +
+                                      (long double)big_vel[3 * i + 1] * big_vel[3 * i + 1] +
+                                      (long double)big_vel[3 * i + 2] * big_vel[3 * i + 2]);
+      big_ke += (double)(0.5 * big_mass * v * v);
+    }
+    if (sf_set_big_beads(h, nbig, big_pos.data(), big_sigma, o.G, o.H)) return fail("sf_set_big_beads");
+  }
   if (sf_set_state(h, pos.data(), vel.data())) return fail("sf_set_state");
   std::vector<double> energies(7), per_bead(5 * N);
   double vcom = 0;
@@ -330,6 +362,13 @@ int main(int argc, char** argv) {
          << ke * 2 / (3 * N) << std::setw(15) << tp / N << std::setw(15) << tk / N;
     if (with_vcom) traj << std::setw(15) << vcom;
     traj << std::endl;
+    if (nbig > 0) {  // src/md.cpp:347-352, 625-630
+      double be2[2];
+      if (sf_big_energies(h, be2)) return fail("sf_big_energies");
+      const double bce = be2[0], bne = be2[1];
+      encapdata << a << std::setw(15) << big_ke / nbig << std::setw(15) << bne / nbig << std::setw(15) << bce / nbig
+                << std::setw(15) << (big_ke + bne + bce) / nbig << std::setw(15) << big_ke * 2 / (3 * nbig) << std::endl;
+    }
     return 0;
   };
   if (energy_row(0, false)) return 1;
@@ -348,7 +387,7 @@ int main(int argc, char** argv) {
   const double nsteps_d = totaltime / o.dt;
   long long a = loopStart;
   double percentagePre = -1;
-  std::ofstream restart_out;
+  std::ofstream restart_out, rb_out;
   while ((double)a < nsteps_d + 1) {
     // run up to the next iteration that writes output (or the last one)
     long long stop = a;
@@ -367,6 +406,17 @@ int main(int argc, char** argv) {
                     << std::setw(25) << std::setprecision(12) << pos[3 * i + 1] << std::setw(25) << std::setprecision(12)
                     << pos[3 * i + 2] << std::setw(25) << std::setprecision(12) << std::endl;
       restart_out.close();
+      if (nbig > 0) {  // src/md.cpp:537-545
+        std::ostringstream rn;
+        rn << "outfiles/rb_" << a << ".out";
+        rb_out.open(rn.str().c_str(), std::ios::out);
+        for (int i = 0; i < nbig; i++)
+          rb_out << big_vel[3 * i] << std::setw(25) << std::setprecision(12) << big_vel[3 * i + 1] << std::setw(25)
+                 << std::setprecision(12) << big_vel[3 * i + 2] << std::setw(25) << std::setprecision(12) << big_pos[3 * i]
+                 << std::setw(25) << std::setprecision(12) << big_pos[3 * i + 1] << std::setw(25) << std::setprecision(12)
+                 << big_pos[3 * i + 2] << std::setw(25) << std::setprecision(12) << std::endl;
+        rb_out.close();
+      }
     }
     if (a % o.W == 0) {  // src/md.cpp:551-623
       sf_stats st;
@@ -390,13 +440,17 @@ int main(int argc, char** argv) {
     }
     if (a % o.M == 0) {  // src/md.cpp:647-669
       sf_get_state(h, pos.data(), nullptr);
-      ofile << "ITEM: TIMESTEP" << std::endl << a << std::endl << "ITEM: NUMBER OF ATOMS" << std::endl << N << std::endl
+      ofile << "ITEM: TIMESTEP" << std::endl << a << std::endl << "ITEM: NUMBER OF ATOMS" << std::endl << N + nbig << std::endl
             << "ITEM: BOX BOUNDS" << std::endl << -prm.box / 2 << std::setw(15) << prm.box / 2 << std::endl << -prm.box / 2
             << std::setw(15) << prm.box / 2 << std::endl << -prm.box / 2 << std::setw(15) << prm.box / 2 << std::endl
             << "ITEM: ATOMS index type x y z b charge" << std::endl;
       for (long long b = 0; b < N; b++)
         ofile << b + 1 << std::setw(15) << tb.type[b % np] << std::setw(15) << pos[3 * b] << std::setw(15) << pos[3 * b + 1]
               << std::setw(15) << pos[3 * b + 2] << std::setw(15) << per_bead[5 * b + 1] << std::setw(15) << tb.q[b % np]
+              << std::endl;
+      for (int i = 0; i < nbig; i++)  // src/md.cpp:671-675 (a big bead's be stays 0)
+        ofile << N + i + 1 << std::setw(15) << big_type << std::setw(15) << big_pos[3 * i] << std::setw(15)
+              << big_pos[3 * i + 1] << std::setw(15) << big_pos[3 * i + 2] << std::setw(15) << 0.0 << std::setw(15) << o.G
               << std::endl;
     }
     const double percentage = std::round(a / (totaltime / o.dt) * 100 * 10) / 10;

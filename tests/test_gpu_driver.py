@@ -77,3 +77,33 @@ def test_driver_oligomers_and_uncut_yukawa(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     rows = _rows(d2 / "outfiles" / "energy.out")
     assert [x[0] for x in rows] == [0, 0, 50, 100] and all(np.isfinite(x).all() for x in rows)
+
+
+def test_driver_encapsulation(tmp_path):
+    """-p: immobile big beads (src/md.cpp:183-185), capenergy.out rows (src/md.cpp:347-352,
+    625-630), rb_<a>.out beside each restart file (src/md.cpp:537-545), the big beads in each
+    movie frame (src/md.cpp:649-676) and the rb restart (src/initialize.cpp:574-600)."""
+    os.makedirs(tmp_path / "infiles")
+    shutil.copy(TEMPLATE, tmp_path / "infiles" / "trimer_MVM")
+    _run(tmp_path, ["-T", "100", "-W", "50", "-w", "50", "-M", "50", "-R", "false", "-p", "true", "-m", "2"])
+    out = tmp_path / "outfiles"
+    rows = _rows(out / "capenergy.out")
+    assert [r[0] for r in rows] == [0, 0, 50, 100]
+    for r in rows:
+        assert len(r) == 6 and np.isfinite(r).all()
+        assert r[1] == 0 and r[5] == 0              # immobile, zero velocity
+        assert r[2] >= 0                            # shifted WCA energy
+        assert abs(r[4] - (r[1] + r[2] + r[3])) <= 1e-5 * max(1.0, abs(r[4]))
+    big = [np.loadtxt(out / f"rb_{a}.out") for a in (0, 50, 100)]
+    for b in big:
+        assert b.shape == (2, 6) and np.array_equal(b, big[0])
+    text = open(out / "ovito.lammpstrj").read()
+    n_sub = 20 * 105
+    assert text.count("ITEM: NUMBER OF ATOMS\n%d\n" % (n_sub + 2)) == 3
+    last = text.strip().splitlines()[-1].split()
+    assert int(last[0]) == n_sub + 2 and int(last[1]) == 3 and float(last[6]) == -78
+    # resume: subunits from restart_100.out, big beads from rb_100.out
+    _run(tmp_path, ["-T", "150", "-W", "50", "-w", "50", "-M", "1000", "-R", "true", "-p", "true", "-m", "2"])
+    rows2 = _rows(out / "capenergy.out")
+    assert [r[0] for r in rows2] == [0, 100, 150]
+    assert np.array_equal(np.loadtxt(out / "rb_150.out"), big[0])
