@@ -144,7 +144,7 @@ struct sf_handle {
   // far field of the uncut Yukawa (split mode only): the model when one is attached (energies
   // always use the exact far field)
   int launch_far(int energy) {
-    if (d.nbig)
+    if (d.nbig && energy)  // big-bead energies; their forces are evaluated inside k_force
       if (int rc = launch_big(d, st)) return rc;
     if (!d.split) return SF_OK;
     if (far_drop && !energy) return launch_fill(d.qfar, 3 * (size_t)d.Nq, 0.0, st);

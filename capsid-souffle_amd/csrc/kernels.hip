@@ -1348,6 +1348,45 @@ static inline size_t force_lds_bytes(int np, int nh, int nq, int tile) {
   return (size_t)np * 32 + (size_t)std::max(nh * 9, tile * 3) * 8 + (size_t)nq * nq * 16;
 }
 
+// Encapsulation big beads on subunit bead p of template index l (src/forces.cpp:174-236 and
+// src/energies.cpp:96-136), big beads in order. The immobile big beads exert a Yukawa force
+// on charged beads within ecut_el and a WCA force shifted by delta = (sigma_big + sigma_j) / 2
+// - shc; each term is subtracted from the bead's sums as there (the reference's vector is big
+// bead minus subunit bead). ENERGY adds the energies (ne, ce) of the same terms.
+template <bool ENERGY>
+__device__ __forceinline__ void big_bead_terms(const Dev& d, const RepParams& P, double4 p, int l, double* lj,
+                                               double* el, double* ne, double* ce) {
+  const double box = P.box, hbox = P.hbox, kappa = P.kappa, ecut_el = P.ecut_el;
+  const double4 t0 = d.big_tab[2 * l], t1 = d.big_tab[2 * l + 1];
+  const double delta = t0.x, wca_cut = t0.y, wca_cut2 = t0.z, ys = t0.w;
+  const double qqlb = t1.x, ecut_el2 = t1.z, shc6 = t1.w;
+  const bool charged = t1.y != 0;
+  for (int i = 0; i < d.nbig; i++) {
+    const double4 c = d.big_pos[i];
+    const double rx = min_img(c.x - p.x, box, hbox), ry = min_img(c.y - p.y, box, hbox),
+                 rz = min_img(c.z - p.z, box, hbox);
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    const bool yk = charged && r2 < ecut_el2, wca = r2 < wca_cut2;
+    if (!(yk || wca || ENERGY)) continue;
+    const double r = sqrt(r2);
+    const double u = r - delta, u2 = u * u, r6 = u2 * u2 * u2;
+    const double x6 = shc6 / r6;
+    const double ek = charged && (yk || r < ecut_el) ? exp(-kappa * r) : 0.0;
+    if (yk) {
+      const double sc = ((qqlb * ek) / r2) * (kappa + 1 / r) * ys;
+      el[0] -= rx * sc; el[1] -= ry * sc; el[2] -= rz * sc;
+    }
+    if (wca) {
+      const double sc = 48 * (x6 * (x6 - 0.5)) * (1 / r) * (1 / u);
+      lj[0] -= rx * sc; lj[1] -= ry * sc; lj[2] -= rz * sc;
+    }
+    if (ENERGY) {
+      if (charged && r < ecut_el) *ce += (qqlb * ek * ys) / r;
+      if (r < wca_cut) *ne += (4 * x6 * (x6 - 1)) + 1;
+    }
+  }
+}
+
 // kChunk: list entries the workgroup evaluates per gather round (the tile).
 template <int MODE, typename PT, int kChunk>
 // Four waves per SIMD is what the LDS allows (8 workgroups per CU): up to 128 VGPRs.
@@ -1549,8 +1588,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     const double elx = exd + (double)lx, ely = eyd + (double)ly, elz = ezd + (double)lz;
     double Fx = sx + bx + elx, Fy = sy + by + ely, Fz = sz + bz + elz;
     if (d.nbig) {  // src/forces.cpp:234-235: tforce + ljforce + eforce of the big-bead terms
-      const double4 bl = d.big_f[2 * (size_t)gi], be = d.big_f[2 * (size_t)gi + 1];
-      Fx = (Fx + bl.x) + be.x; Fy = (Fy + bl.y) + be.y; Fz = (Fz + bl.z) + be.z;
+      double bl[3] = {0, 0, 0}, be[3] = {0, 0, 0};
+      big_bead_terms<false>(d, P, own[tid], tid, bl, be, nullptr, nullptr);
+      Fx = (Fx + bl[0]) + be[0]; Fy = (Fy + bl[1]) + be[1]; Fz = (Fz + bl[2]) + be[2];
     }
     if (MODE == MODE_FORCES) {
       if (d.out_f) { d.out_f[3 * (size_t)gi] = Fx; d.out_f[3 * (size_t)gi + 1] = Fy; d.out_f[3 * (size_t)gi + 2] = Fz; }
@@ -1927,44 +1967,17 @@ __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
   if (tid < 5) d.epart[(size_t)s * 5 + tid] = red[0][tid] + red[1][tid];
 }
 
-// Encapsulation big beads on the subunit beads (src/forces.cpp:174-236) and their energies
-// (src/energies.cpp:96-136): one thread per subunit bead, big beads in order. The immobile
-// big beads exert a Yukawa force on charged beads within ecut_el and a WCA force shifted by
-// delta = (sigma_big + sigma_j) / 2 - shc; each term is subtracted from the bead's running
-// sums as there (the reference's vector is big bead minus subunit bead).
+// Big-bead terms of every subunit bead with their energies, for the energy entries: one thread
+// per bead; out[2b] = (ljforce, ne), out[2b + 1] = (eforce, ce). The step and force paths
+// evaluate the forces inside k_force instead.
 __global__ __launch_bounds__(256) void k_big(Dev d) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= d.Nb) return;
   const double4 p = d.pos[b];
-  const int l = tag_loc(p.w);
-  const RepParams& P = d.rp[0];
-  const double box = P.box, hbox = P.hbox, kappa = P.kappa, ecut_el = P.ecut_el;
-  const double4 t0 = d.big_tab[2 * l], t1 = d.big_tab[2 * l + 1];
-  const double delta = t0.x, wca_cut = t0.y, wca_cut2 = t0.z, ys = t0.w;
-  const double qqlb = t1.x, ecut_el2 = t1.z, shc6 = t1.w;
-  const bool charged = t1.y != 0;
-  double lx = 0, ly = 0, lz = 0, ex = 0, ey = 0, ez = 0, ne = 0, ce = 0;
-  for (int i = 0; i < d.nbig; i++) {
-    const double4 c = d.big_pos[i];
-    const double rx = min_img(c.x - p.x, box, hbox), ry = min_img(c.y - p.y, box, hbox),
-                 rz = min_img(c.z - p.z, box, hbox);
-    const double r2 = rx * rx + ry * ry + rz * rz, r = sqrt(r2);
-    const double u = r - delta, u2 = u * u, r6 = u2 * u2 * u2;
-    const double x6 = shc6 / r6;
-    const double ek = charged && (r2 < ecut_el2 || r < ecut_el) ? exp(-kappa * r) : 0.0;
-    if (charged && r2 < ecut_el2) {
-      const double sc = ((qqlb * ek) / r2) * (kappa + 1 / r) * ys;
-      ex -= rx * sc; ey -= ry * sc; ez -= rz * sc;
-    }
-    if (r2 < wca_cut2) {
-      const double sc = 48 * (x6 * (x6 - 0.5)) * (1 / r) * (1 / u);
-      lx -= rx * sc; ly -= ry * sc; lz -= rz * sc;
-    }
-    if (charged && r < ecut_el) ce += (qqlb * ek * ys) / r;
-    if (r < wca_cut) ne += (4 * x6 * (x6 - 1)) + 1;
-  }
-  d.big_f[2 * (size_t)b] = make_double4(lx, ly, lz, ne);
-  d.big_f[2 * (size_t)b + 1] = make_double4(ex, ey, ez, ce);
+  double lj[3] = {0, 0, 0}, el[3] = {0, 0, 0}, ne = 0, ce = 0;
+  big_bead_terms<true>(d, d.rp[0], p, tag_loc(p.w), lj, el, &ne, &ce);
+  d.big_f[2 * (size_t)b] = make_double4(lj[0], lj[1], lj[2], ne);
+  d.big_f[2 * (size_t)b + 1] = make_double4(el[0], el[1], el[2], ce);
 }
 
 __global__ void k_bd_uniforms(Dev d, uint32_t state, int nb, double* out) {

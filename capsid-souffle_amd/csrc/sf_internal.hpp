@@ -185,8 +185,8 @@ struct Dev {
   int tile;            // list entries k_force stages in LDS per gather round (128, 256, 384, 512)
   // encapsulation big beads (single-replica handles): immobile positions, per-template-bead
   // constants [2 np] = (delta, wca_cut, wca_cut^2, yukawa_scale), (q_big q lb, charged,
-  // ecut_el^2, shc^6), and the big-bead terms of each subunit bead [2 Nb] = (ljforce, ne),
-  // (eforce, ce)
+  // ecut_el^2, shc^6), and, for the energy entries, the big-bead terms of each subunit bead
+  // [2 Nb] = (ljforce, ne), (eforce, ce)
   int nbig;
   const double4* big_pos;
   const double4* big_tab;
