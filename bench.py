@@ -72,6 +72,10 @@ def extended_energy(s):
 
 def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
     s, pos, vel = make_system(args, brownian, fp32, skin)
+    if args.no_graph:
+        # direct kernel launches throughout (the library launches event-timed chunks without a
+        # graph); for kernel traces of the step kernels
+        s.set_timing(True)
     e_start = extended_energy(s)
     tune = None
     s.step(args.warmup)
@@ -97,7 +101,8 @@ def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
         # per-kernel means: HIP events around every 10th step's kernels (direct launches)
         s.set_timing(True)
         s.step(min(args.steps, args.event_steps))
-        s.set_timing(False)
+        if not args.no_graph:
+            s.set_timing(False)
         out["stats"] = s.stats()
     out["system"] = s
     return out
@@ -159,6 +164,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the BD and FP32-pair runs")
     ap.add_argument("--probe-copy-mb", type=int, default=0, help="run a known-size copy kernel first")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the step kernels directly instead of replaying HIP graphs (for kernel traces)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"replicas": 3000, "surrogate": 1000}.get(args.workload, 20000)
@@ -384,6 +391,9 @@ def bench_md(args):
                                    measured_traffic(args)),
         "kernels_ms": {"k_integrate": st["k_integrate_ms"], "k_rebuild": st["k_rebuild_ms"],
                        "k_force": t_force, "k_finalize": st["k_reduce_ms"]},
+        # the event-timed pass launches the step kernels directly; each mean also holds the
+        # cost of its two event records (~3 us against a kernel trace of the same launches)
+        "kernel_ms_note": "HIP-event means over direct launches, event cost included",
     }
     if not args.no_extra:
         import numpy as np
