@@ -112,3 +112,30 @@ def test_step_needs_restart_after_big_bead_change():
         sysg.step(1)
     sysg.set_state(pos, vel)
     sysg.step(1)
+
+
+def test_removing_big_beads_restores_the_plain_path():
+    """n = 0 removes the big beads: the run is then bitwise the run of a handle that never had
+    them (forces, noise stream multiplier and graph all back to the plain path)."""
+    outs = []
+    for with_then_without in (False, True):
+        t, p, sysg = _system(20, brownian=True)
+        o = _oracle(20)
+        pos, vel, c = _setup(14, p.box, o)
+        if with_then_without:
+            sysg.set_big_beads(c, **BIG)
+            sysg.set_state(pos, vel)
+            sysg.step(20)
+            sysg.set_big_beads(np.zeros((0, 3)), **BIG)
+        sysg.set_state(pos, vel)
+        sysg.step(50)
+        outs.append(sysg.get_state())
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_big_beads_need_a_single_replica_handle():
+    t = sg.Template(TEMPLATE)
+    p = t.params(**dict(CFG0, n_subunits=20, q_chain=5))
+    batch = sg.System(t, [20, 20], [p, p])
+    with pytest.raises(sg.SfError):
+        batch.set_big_beads(np.zeros((1, 3)), **BIG)
