@@ -14,6 +14,9 @@
 // oligomer whatever order the unions ran in.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "sf_internal.hpp"
 
 namespace sf {
@@ -65,12 +68,16 @@ __global__ void k_olig_extent(Dev d, unsigned long long* ext) {
   atomicMax(ext + d.sub_rep[s], (unsigned long long)__double_as_longlong(r2));
 }
 
-__global__ void k_olig_init(int ns, int* parent, int* size, int* spectrum) {
+// Resets the union-find and the counts; ext (one slot per replica, R <= ns) and the total are
+// cleared here too, so the analysis issues no memset.
+__global__ void k_olig_init(int ns, int R, int* parent, int* size, int* spectrum, unsigned long long* ext, int* total) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= ns) return;
   parent[s] = s;
   size[s] = 0;
   spectrum[s] = 0;
+  if (s < R) ext[s] = 0ull;
+  if (s == 0) *total = 0;
 }
 
 __global__ __launch_bounds__(kBlock) void k_olig_contacts(Dev d, double contact, const unsigned long long* ext,
@@ -132,31 +139,82 @@ __global__ void k_olig_label(Dev d, const int* parent, int* label, int* size) {
 }
 
 // spectrum[s0 + k - 1] = oligomers of k subunits in the replica starting at subunit s0
-__global__ void k_olig_spectrum(Dev d, const int* label, const int* size, int* spectrum) {
+__global__ void k_olig_spectrum(Dev d, const int* label, const int* size, int* spectrum, int* total) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.Ns) return;
   const int s0 = d.rp[d.sub_rep[s]].s0;
-  if (label[s] == s - s0) atomicAdd(spectrum + s0 + size[s] - 1, 1);
+  if (label[s] == s - s0) {
+    atomicAdd(spectrum + s0 + size[s] - 1, 1);
+    atomicAdd(total, 1);
+  }
 }
 
 }  // namespace
 
+// Fixed device buffers of the allocation-free entry point (sf_oligomers_noalloc): ext takes
+// the first kOligoScratchSubunits words (R <= Ns), then parent, size, label and spectrum
+// (kOligoScratchSubunits ints each), then the total. Every GPU the library runs on holds its own
+// copy until the process ends. In host code `scratch` names a host-side stand-in, not device
+// memory: kernels and copies get the device address from hipGetSymbolAddress instead.
+constexpr size_t kOligoScratchWords = 3 * (size_t)kOligoScratchSubunits + 32;
+static __device__ __align__(256) unsigned long long scratch[kOligoScratchWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];  // device address of `scratch` on each device, once resolved
+}  // namespace
+
 int launch_oligomers(const Dev& d, double contact, OligoBuf& o, hipStream_t st) {
   const int nb_blocks = (d.Nb + 255) / 256, ns_blocks = (d.Ns + 255) / 256;
-  if (hipMemsetAsync(o.ext, 0, (size_t)d.R * sizeof(unsigned long long), st) != hipSuccess) {
-    set_error("hipMemsetAsync failed (oligomers)");
-    return SF_E_HIP;
-  }
+  k_olig_init<<<ns_blocks, 256, 0, st>>>(d.Ns, d.R, o.parent, o.size, o.spectrum, o.ext, o.total);
   k_olig_extent<<<nb_blocks, 256, 0, st>>>(d, o.ext);
-  k_olig_init<<<ns_blocks, 256, 0, st>>>(d.Ns, o.parent, o.size, o.spectrum);
   k_olig_contacts<<<d.Ns, kBlock, 0, st>>>(d, contact, o.ext, o.parent);
   k_olig_label<<<ns_blocks, 256, 0, st>>>(d, o.parent, o.label, o.size);
-  k_olig_spectrum<<<ns_blocks, 256, 0, st>>>(d, o.label, o.size, o.spectrum);
+  k_olig_spectrum<<<ns_blocks, 256, 0, st>>>(d, o.label, o.size, o.spectrum, o.total);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error(std::string("oligomer kernels: ") + hipGetErrorString(e));
     return SF_E_HIP;
   }
+  return SF_OK;
+}
+
+int oligo_scratch(int dev, int ns, int R, OligoBuf& o) {
+  if (ns <= 0 || R <= 0 || R > ns) { set_error("bad handle sizes (oligomers)"); return SF_E_ARG; }
+  if (ns > kOligoScratchSubunits) {
+    set_error("sf_oligomers_noalloc serves at most " + std::to_string(kOligoScratchSubunits) +
+              " subunits per handle; this one has " + std::to_string(ns) + " (use sf_oligomers)");
+    return SF_E_ARG;
+  }
+  if (dev < 0 || dev >= kMaxDevices) { set_error("device index out of range (oligomers)"); return SF_E_ARG; }
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      // the address is per device: resolve it with that device current
+      int cur = -1;
+      if (hipGetDevice(&cur) != hipSuccess) { set_error("hipGetDevice failed (oligomers)"); return SF_E_HIP; }
+      if (cur != dev && hipSetDevice(dev) != hipSuccess) { set_error("hipSetDevice failed (oligomers)"); return SF_E_HIP; }
+      void* ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (cur != dev) (void)hipSetDevice(cur);
+      if (e != hipSuccess || !ptr) {
+        set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        return SF_E_HIP;
+      }
+      g_scratch_ptr[dev] = ptr;
+    }
+    base = g_scratch_ptr[dev];
+  }
+  unsigned long long* w = static_cast<unsigned long long*>(base);
+  int* ints = reinterpret_cast<int*>(w + kOligoScratchSubunits);
+  o.ext = w;
+  o.parent = ints;
+  o.size = ints + kOligoScratchSubunits;
+  o.label = ints + 2 * (size_t)kOligoScratchSubunits;
+  o.spectrum = ints + 3 * (size_t)kOligoScratchSubunits;
+  o.total = ints + 4 * (size_t)kOligoScratchSubunits;
   return SF_OK;
 }
 

@@ -36,6 +36,7 @@ constexpr double kNoCut2 = 1e30;     // squared "cutoff" of the uncut Yukawa's b
 struct sf_handle {
   sf_template t;
   int R = 0, Ns = 0, Nb = 0, Nq = 0;
+  int dev = 0;                 // HIP device current at sf_create; the handle's memory lives there
   std::vector<int> ns;
   std::vector<sf_params> prm;
   std::vector<RepParams> rp;
@@ -488,6 +489,7 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
     delete h;
     return nullptr;
   };
+  if (hipGetDevice(&h->dev) != hipSuccess) return fail("hipGetDevice failed (no GPU?)");
   if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) return fail("hipStreamCreate failed (no GPU?)");
   Dev& d = h->d;
   d.Nb = h->Nb; d.Ns = h->Ns; d.Nq = h->Nq; d.R = R;
@@ -1114,7 +1116,7 @@ int sf_oligomers(sf_handle* h, double contact, int* label, int* mass_spectrum) {
   OligoBuf& o = h->olig;
   if (!o.parent) {
     if (h->alloc(&o.ext, h->R) || h->alloc(&o.parent, h->Ns) || h->alloc(&o.size, h->Ns) ||
-        h->alloc(&o.label, h->Ns) || h->alloc(&o.spectrum, h->Ns))
+        h->alloc(&o.label, h->Ns) || h->alloc(&o.spectrum, h->Ns) || h->alloc(&o.total, 1))
       return SF_E_HIP;
   }
   if (int rc = launch_centres(h->d, h->st)) return rc;
@@ -1128,6 +1130,31 @@ int sf_oligomers(sf_handle* h, double contact, int* label, int* mass_spectrum) {
   int n = 0;
   for (int s = 0; s < h->Ns; s++) n += ms[s];
   return n;
+}
+
+int sf_oligomers_noalloc(sf_handle* h, double contact, int* label, int* mass_spectrum) {
+  if (!h) { set_error("null handle"); return SF_E_ARG; }
+  if (!(contact > 0) || !std::isfinite(contact)) { set_error("contact distance must be positive"); return SF_E_ARG; }
+  OligoBuf o{};
+  if (int rc = oligo_scratch(h->dev, h->Ns, h->R, o)) return rc;
+  int cur = -1;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != h->dev) HIPCHK(hipSetDevice(h->dev));
+  int rc = launch_centres(h->d, h->st);
+  if (!rc) rc = launch_oligomers(h->d, contact, o, h->st);
+  int n = -1;
+  if (!rc) {
+    // straight into the caller's arrays: no staging buffer
+    hipError_t e = hipSuccess;
+    if (label) e = hipMemcpyAsync(label, o.label, h->Ns * sizeof(int), hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess && mass_spectrum)
+      e = hipMemcpyAsync(mass_spectrum, o.spectrum, h->Ns * sizeof(int), hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&n, o.total, sizeof(int), hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    if (e != hipSuccess) { set_error(std::string("sf_oligomers_noalloc: ") + hipGetErrorString(e)); rc = SF_E_HIP; }
+  }
+  if (cur != h->dev) (void)hipSetDevice(cur);
+  return rc ? rc : n;
 }
 
 }  // extern "C"

@@ -203,7 +203,12 @@ struct OligoBuf {
   int* size;                // [Ns] oligomer size at its root
   int* label;               // [Ns] replica-local index of the oligomer's smallest subunit
   int* spectrum;            // [Ns] per replica (at its first subunit): oligomers by size
+  int* total;               // [1] oligomers over all replicas
 };
+
+// Largest handle (total subunits) the allocation-free oligomer entry point serves: its
+// buffers live in a fixed device array of analysis.hip (6 MB + 256 B per GPU).
+constexpr int kOligoScratchSubunits = 1 << 18;
 
 // kernel launchers (kernels.hip)
 int launch_centres(const Dev& d, hipStream_t st);
@@ -219,6 +224,9 @@ int launch_energy(const Dev& d, hipStream_t st);
 int launch_big(const Dev& d, hipStream_t st);
 int launch_bd_uniforms_state(uint32_t state, int nb, double* out_dev, const Dev& d, hipStream_t st);
 int launch_oligomers(const Dev& d, double contact, OligoBuf& o, hipStream_t st);  // analysis.hip
+// Carves OligoBuf for ns subunits and R replicas out of device `dev`'s fixed scratch array
+// (analysis.hip); the array's address is resolved once per device. No allocation.
+int oligo_scratch(int dev, int ns, int R, OligoBuf& o);
 
 // host helpers (topology.cpp)
 uint32_t lcg_pow(uint64_t e);   // 16807^e mod (2^31-1)

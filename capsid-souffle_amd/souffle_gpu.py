@@ -111,6 +111,7 @@ SIGNATURES = {
     "sf_far_drop": (ctypes.c_int, [_P, ctypes.c_int]),
     "sf_get_far_stats": (ctypes.c_int, [_P, ctypes.POINTER(SfFarStats)]),
     "sf_oligomers": (ctypes.c_int, [_P, ctypes.c_double, _I, _I]),
+    "sf_oligomers_noalloc": (ctypes.c_int, [_P, ctypes.c_double, _I, _I]),
     "sf_big_beads_place": (ctypes.c_int, [_P, ctypes.c_int, _D, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_uint, _D]),
     "sf_set_big_beads": (ctypes.c_int, [_P, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
@@ -325,6 +326,20 @@ class System:
             out.append((lab[s0:s0 + n].copy(), ms[s0:s0 + n].copy()))
             s0 += n
         return out
+
+    def oligomers_noalloc(self, contact=2.0):
+        """As oligomers(), through the allocation-free entry point (fixed device buffers, at most
+        262,144 subunits per handle); also returns the oligomer count it reports."""
+        ns = sum(self.n_subunits)
+        lab = np.zeros(ns, dtype=np.int32)
+        ms = np.zeros(ns, dtype=np.int32)
+        n = _err(_lib.sf_oligomers_noalloc(self.ptr, float(contact), lab.ctypes.data_as(_I), ms.ctypes.data_as(_I)),
+                 "sf_oligomers_noalloc")
+        out, s0 = [], 0
+        for k in self.n_subunits:
+            out.append((lab[s0:s0 + k].copy(), ms[s0:s0 + k].copy()))
+            s0 += k
+        return out, n
 
     def set_big_beads(self, xyz, sigma=6.0, charge=-78.0, shc=3.5):
         """Immobile encapsulation big beads at xyz[n][3] (defaults: the reference's -a 3, -G -78,

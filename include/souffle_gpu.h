@@ -305,6 +305,16 @@ int sf_get_far_stats(sf_handle* h, sf_far_stats* out);
  * Returns the total number of oligomers over all replicas, or a negative error code. */
 int sf_oligomers(sf_handle* h, double contact, int* label, int* mass_spectrum);
 
+/* The same analysis with no device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Outputs and return value are those of sf_oligomers.
+ * The union-find buffers come from a fixed array in the library's device code (one per GPU,
+ * about 6 MB, held until the process ends) instead of memory the handle allocates; handles of
+ * more than 262,144 subunits in total are refused with SF_E_ARG. Every call on one GPU shares
+ * that GPU's array (the call clears what it uses and waits for its results before returning),
+ * so calls that overlap on one GPU, from several host threads, race. */
+int sf_oligomers_noalloc(sf_handle* h, double contact, int* label, int* mass_spectrum);
+
 /* ---- encapsulation big beads ------------------------------------------------------------ */
 
 /* Places n big beads of diameter sigma among the subunit beads pos[n_subunits * n_beads][3] of
