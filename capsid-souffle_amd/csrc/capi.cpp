@@ -52,9 +52,10 @@ struct sf_handle {
   bool far_fell_back = false;
   double far_tol = 1e-3;
   long long far_audits = 0;
-  double far_last = 0, far_max = 0;
+  double far_last = 0, far_max = 0;            // audit error relative to the far field
+  double far_last_force = 0, far_max_force = 0;  // the same difference relative to the total force
   double* qfar_exact = nullptr;  // [Nq][3] exact far field at audits
-  double* audit_out = nullptr;   // [2]
+  double* audit_out = nullptr;   // [R][3]
   int mode = MODE_MD;
   int fp32 = 0;
   bool timing = false;
@@ -157,19 +158,30 @@ struct sf_handle {
   }
 
   // relative error of the model's far field (in d.qfar after the last force evaluation)
-  // against the exact one at the current positions, over the force of all beads
+  // against the exact one at the current positions: per replica
+  // sqrt(sum |F_model - F_exact|^2 / sum |F_exact|^2) over its charged beads, i.e. relative to
+  // the far field the model replaces; the largest over the replicas
   int audit(double* err) {
     Dev dx = d;
     dx.qfar = qfar_exact;
     if (int rc = launch_far_exact(dx, 0, st)) return rc;
     if (int rc = launch_far_audit(d, qfar_exact, audit_out, st)) return rc;
-    double h2[2];
-    HIPCHK(hipMemcpyAsync(h2, audit_out, sizeof(h2), hipMemcpyDeviceToHost, st));
+    std::vector<double> h3(3 * (size_t)R);
+    HIPCHK(hipMemcpyAsync(h3.data(), audit_out, h3.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    *err = std::sqrt(h2[0] / std::max(h2[1], 1e-300));
+    double worst = 0.0, worst_f = 0.0;
+    auto ratio = [](double a, double b) { return b > 0 ? std::sqrt(a / b) : (a > 0 ? HUGE_VAL : 0.0); };
+    for (int r = 0; r < R; r++) {
+      const double e = ratio(h3[3 * r], h3[3 * r + 1]), ef = ratio(h3[3 * r], h3[3 * r + 2]);
+      if (!(e <= worst)) worst = e;  // a NaN error counts as the worst
+      if (!(ef <= worst_f)) worst_f = ef;
+    }
+    *err = worst;
     far_audits++;
-    far_last = *err;
-    far_max = std::max(far_max, *err);
+    far_last = worst;
+    far_max = std::max(far_max, worst);
+    far_last_force = worst_f;
+    far_max_force = std::max(far_max_force, worst_f);
     return SF_OK;
   }
 
@@ -264,11 +276,19 @@ struct sf_handle {
     return SF_OK;
   }
 
-  // After a launch sequence: inspect overflow / deformation flags. Returns 1 if the sequence
-  // must be replayed (capacities or bounds were raised), 0 if fine, <0 on error.
+  // After a launch sequence: inspect non-finite, overflow and deformation flags. Returns 2 if a
+  // replica became non-finite (bad_step, bad_rep: the first such step and its replica; nothing
+  // is regrown, since a non-finite state can ask for any capacity), 1 if the sequence must be
+  // replayed (capacities or bounds were raised), 0 if fine, <0 on error.
+  long long bad_step = 0;
+  int bad_rep = -1;
   int check_and_fix() {
     std::vector<RepState> h;
     if (int rc = read_states(h)) return rc;
+    bad_rep = -1;
+    for (int r = 0; r < R; r++)
+      if (h[r].nonfinite && (bad_rep < 0 || h[r].bad_step < bad_step)) { bad_rep = r; bad_step = h[r].bad_step; }
+    if (bad_rep >= 0) return 2;
     int need_lj = 0, need_q = 0, need_c = 0, deformed = 0, need_clj = 0, need_cq = 0;
     for (auto& s : h) {
       need_lj = std::max(need_lj, s.need_lj);
@@ -334,6 +354,10 @@ struct sf_handle {
       int rc = check_and_fix();
       if (rc < 0) return rc;
       if (rc == 0) return SF_OK;
+      if (rc == 2) {
+        set_error("non-finite initial state (replica " + std::to_string(bad_rep) + ")");
+        return SF_E_NONFINITE;
+      }
       std::vector<RepState> h;
       if ((rc = read_states(h))) return rc;
       for (auto& s : h) { s.need_lj = s.need_q = s.need_cand = s.deformed = s.need_cell_lj = s.need_cell_q = 0; }
@@ -365,10 +389,12 @@ struct sf_handle {
     return SF_OK;
   }
 
-  // The chunk is captured once into a HIP graph and replayed; the capture is redone when the
-  // chunk length, the timing mode or any list buffer (reallocated on growth) changes.
-  hipGraphExec_t gexec = nullptr;
-  int g_n = -1;
+  // Each chunk length is captured once into a HIP graph and replayed (a few lengths are kept:
+  // the full chunk and the remainders callers use); every capture is redone when the timing
+  // mode, the tile, the far-field form or any list buffer (reallocated on growth) changes.
+  static constexpr int kGraphCache = 4;
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;  // (chunk length, executable), oldest first
+  bool g_valid = false;
   bool g_tm = false;
   const void* g_key[2] = {nullptr, nullptr};
   int g_cap[2] = {0, 0};
@@ -389,11 +415,10 @@ struct sf_handle {
   // The executable graph's last launch may still be running on the stream: it is drained before
   // the graph is destroyed (HIP does not promise to defer the destruction of a running graph).
   void drop_graph() {
-    if (gexec) {
-      (void)hipStreamSynchronize(st);
-      (void)hipGraphExecDestroy(gexec);
-    }
-    gexec = nullptr;
+    if (!graphs.empty()) (void)hipStreamSynchronize(st);
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+    g_valid = false;
   }
   int ensure_events() {
     const size_t need = kEv * ((kChunk + kTimeEvery - 1) / kTimeEvery);
@@ -412,26 +437,38 @@ struct sf_handle {
     }
     // the captured kernel arguments hold the list buffers and their capacities; a regrown
     // buffer may come back at the same address, so the capacities are part of the key
-    if (!gexec || g_n != n || g_tm != timing || g_key[0] != d.lj_list || g_key[1] != d.q_list ||
+    if (!g_valid || g_tm != timing || g_key[0] != d.lj_list || g_key[1] != d.q_list ||
         g_cap[0] != d.lj_cap || g_cap[1] != d.q_cap || g_tile != d.tile || g_far != far_model || g_nbig != d.nbig ||
         g_drop != far_drop || g_cell[0] != d.cell_lj || g_cell[1] != d.cnt_lj || g_cell[2] != d.cell_q ||
         g_cell[3] != d.cnt_q || g_ccap[0] != d.ccap_lj || g_ccap[1] != d.ccap_q) {
       drop_graph();
+      g_tm = timing; g_key[0] = d.lj_list; g_key[1] = d.q_list;
+      g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap; g_tile = d.tile; g_far = far_model; g_drop = far_drop; g_nbig = d.nbig;
+      g_cell[0] = d.cell_lj; g_cell[1] = d.cnt_lj; g_cell[2] = d.cell_q; g_cell[3] = d.cnt_q;
+      g_ccap[0] = d.ccap_lj; g_ccap[1] = d.ccap_q;
+      g_valid = true;
+    }
+    hipGraphExec_t ex = nullptr;
+    for (auto& g : graphs)
+      if (g.first == n) ex = g.second;
+    if (!ex) {
+      if ((int)graphs.size() >= kGraphCache) {  // the oldest length goes (after its last launch)
+        HIPCHK(hipStreamSynchronize(st));
+        (void)hipGraphExecDestroy(graphs.front().second);
+        graphs.erase(graphs.begin());
+      }
       hipGraph_t g = nullptr;
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       int rc = run_chunk(n, timing);
       hipError_t ec = hipStreamEndCapture(st, &g);
       if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
       if (ec != hipSuccess) { set_error(std::string("graph capture failed: ") + hipGetErrorString(ec)); return SF_E_HIP; }
-      hipError_t ei = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
-      if (ei != hipSuccess) { gexec = nullptr; set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return SF_E_HIP; }
-      g_n = n; g_tm = timing; g_key[0] = d.lj_list; g_key[1] = d.q_list;
-      g_cap[0] = d.lj_cap; g_cap[1] = d.q_cap; g_tile = d.tile; g_far = far_model; g_drop = far_drop; g_nbig = d.nbig;
-      g_cell[0] = d.cell_lj; g_cell[1] = d.cnt_lj; g_cell[2] = d.cell_q; g_cell[3] = d.cnt_q;
-      g_ccap[0] = d.ccap_lj; g_ccap[1] = d.ccap_q;
+      if (ei != hipSuccess) { set_error(std::string("graph instantiate failed: ") + hipGetErrorString(ei)); return SF_E_HIP; }
+      graphs.emplace_back(n, ex);
     }
-    HIPCHK(hipGraphLaunch(gexec, st));
+    HIPCHK(hipGraphLaunch(ex, st));
     return SF_OK;
   }
 };
@@ -632,8 +669,12 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
 
 void sf_destroy(sf_handle* h) { delete h; }
 
-int sf_set_state(sf_handle* h, const double* pos, const double* vel) {
-  if (!h || !pos) { set_error("null argument"); return SF_E_ARG; }
+}  // extern "C"
+
+namespace {
+
+// Positions [N][3] as the device's double4 records (x, y, z, tag).
+std::vector<double4> tagged_positions(const sf_handle* h, const double* pos) {
   const int np = h->t.np;
   const size_t nb = h->Nb;
   std::vector<double4> hp(nb);
@@ -641,23 +682,106 @@ int sf_set_state(sf_handle* h, const double* pos, const double* vel) {
     int s = (int)(i / np), l = (int)(i % np);
     hp[i] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], make_tag(s, l));
   }
+  return hp;
+}
+
+// Uploads a state and starts every replica: from scratch (res == nullptr: thermostat zero, BD
+// engine seeded, step 0) or from a resume record per replica.
+int set_state_impl(sf_handle* h, const double* pos, const double* vel, const sf_resume* res, const double* tforce) {
+  const size_t nb = h->Nb;
+  std::vector<double4> hp = tagged_positions(h, pos);
   std::vector<double> hv(3 * nb, 0.0);
   if (vel)
     for (size_t i = 0; i < nb; i++) {
       hv[i] = vel[3 * i]; hv[nb + i] = vel[3 * i + 1]; hv[2 * nb + i] = vel[3 * i + 2];
     }
+  std::vector<RepState> rs(h->R);
+  std::memset(rs.data(), 0, h->R * sizeof(RepState));
+  for (int r = 0; r < h->R; r++) {
+    rs[r].rng = h->rp[r].seed;
+    rs[r].expfac = 1.0;
+    if (!res) continue;
+    const sf_resume& x = res[r];
+    const RepParams& P = h->rp[r];
+    if (x.step < 0 || (!P.brownian && x.q_chain != P.q_chain)) {
+      set_error("resume record " + std::to_string(r) + " does not fit the replica (step or chain length)");
+      return SF_E_ARG;
+    }
+    if (P.brownian && (x.rng == 0 || x.rng >= kLcgMod)) {
+      set_error("resume record " + std::to_string(r) + ": BD engine state must be in 1 .. 2^31-2");
+      return SF_E_ARG;
+    }
+    if (!std::isfinite(x.ke)) { set_error("resume record with a non-finite kinetic energy"); return SF_E_ARG; }
+    rs[r].step = x.step;
+    if (P.brownian) rs[r].rng = x.rng;
+    for (int j = 0; j < P.q_chain; j++) { rs[r].xi[j] = x.xi[j]; rs[r].eta[j] = x.eta[j]; }
+    rs[r].ke = x.ke;
+    rs[r].resume = 1;
+  }
   HIPCHK(hipMemcpyAsync(h->d.pos, hp.data(), nb * sizeof(double4), hipMemcpyHostToDevice, h->st));
   HIPCHK(hipMemcpyAsync(h->d.vx, hv.data(), 3 * nb * sizeof(double), hipMemcpyHostToDevice, h->st));
   HIPCHK(hipMemsetAsync(h->d.fx, 0, 3 * nb * sizeof(double), h->st));
   HIPCHK(hipMemsetAsync(h->d.gx, 0, 3 * nb * sizeof(double), h->st));
-  std::vector<RepState> rs(h->R);
-  std::memset(rs.data(), 0, h->R * sizeof(RepState));
-  for (int r = 0; r < h->R; r++) { rs[r].rng = h->rp[r].seed; rs[r].expfac = 1.0; }
   HIPCHK(hipMemcpyAsync(h->d.rs, rs.data(), h->R * sizeof(RepState), hipMemcpyHostToDevice, h->st));
   int rc = h->start_sequence();
   if (rc) return rc;
+  if (tforce) {  // the integrator force of the interrupted run (BD: with its noise and drag)
+    std::vector<double> hf(3 * nb);
+    for (size_t i = 0; i < nb; i++) {
+      hf[i] = tforce[3 * i]; hf[nb + i] = tforce[3 * i + 1]; hf[2 * nb + i] = tforce[3 * i + 2];
+    }
+    HIPCHK(hipMemcpyAsync(h->d.fx, hf.data(), 3 * nb * sizeof(double), hipMemcpyHostToDevice, h->st));
+  }
   HIPCHK(hipStreamSynchronize(h->st));
   h->big_dirty = false;
+  return SF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sf_set_state(sf_handle* h, const double* pos, const double* vel) {
+  if (!h || !pos) { set_error("null argument"); return SF_E_ARG; }
+  return set_state_impl(h, pos, vel, nullptr, nullptr);
+}
+
+int sf_set_state_resume(sf_handle* h, const double* pos, const double* vel, const sf_resume* res,
+                        const double* tforce) {
+  if (!h || !pos || !vel || !res) { set_error("null argument"); return SF_E_ARG; }
+  if (h->mode == MODE_BD && !tforce) { set_error("a BD resume needs the integrator force (tforce)"); return SF_E_ARG; }
+  return set_state_impl(h, pos, vel, res, tforce);
+}
+
+int sf_get_resume(sf_handle* h, sf_resume* out, double* tforce) {
+  if (!h || !out) { set_error("null argument"); return SF_E_ARG; }
+  std::vector<RepState> s;
+  if (int rc = h->read_states(s)) return rc;
+  for (int r = 0; r < h->R; r++) {
+    sf_resume& x = out[r];
+    std::memset(&x, 0, sizeof(x));
+    x.step = s[r].step;
+    x.q_chain = h->rp[r].q_chain;
+    x.rng = s[r].rng;
+    x.ke = s[r].ke;
+    for (int j = 0; j < h->rp[r].q_chain; j++) { x.xi[j] = s[r].xi[j]; x.eta[j] = s[r].eta[j]; }
+  }
+  if (tforce) {
+    const size_t nb = h->Nb;
+    std::vector<double> hf(3 * nb);
+    HIPCHK(hipMemcpyAsync(hf.data(), h->d.fx, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (size_t i = 0; i < nb; i++) { tforce[3 * i] = hf[i]; tforce[3 * i + 1] = hf[nb + i]; tforce[3 * i + 2] = hf[2 * nb + i]; }
+  }
+  return SF_OK;
+}
+
+int sf_set_positions(sf_handle* h, const double* pos) {
+  if (!h || !pos) { set_error("null argument"); return SF_E_ARG; }
+  std::vector<double4> hp = tagged_positions(h, pos);
+  HIPCHK(hipMemcpyAsync(h->d.pos, hp.data(), h->Nb * sizeof(double4), hipMemcpyHostToDevice, h->st));
+  if (int rc = launch_moved(h->d, h->st)) return rc;
+  HIPCHK(hipStreamSynchronize(h->st));
   return SF_OK;
 }
 
@@ -692,6 +816,10 @@ int sf_get_thermostat(sf_handle* h, int replica, double* xi, double* eta) {
 }
 
 int sf_forces(sf_handle* h, double* f, double* fs, double* fb, double* fe, double* flj) {
+  return sf_forces_ex(h, 1, f, fs, fb, fe, flj);
+}
+
+int sf_forces_ex(sf_handle* h, int force_rebuild, double* f, double* fs, double* fb, double* fe, double* flj) {
   if (!h) { set_error("null handle"); return SF_E_ARG; }
   const size_t n3 = 3 * (size_t)h->Nb;
   if (!h->out_buf && h->alloc(&h->out_buf, 5 * n3)) return SF_E_HIP;
@@ -700,7 +828,8 @@ int sf_forces(sf_handle* h, double* f, double* fs, double* fb, double* fe, doubl
     d.out_f = h->out_buf; d.out_s = h->out_buf + n3; d.out_b = h->out_buf + 2 * n3;
     d.out_e = h->out_buf + 3 * n3; d.out_lj = h->out_buf + 4 * n3;
     if (int rc = launch_centres(d, h->st)) return rc;
-    if (int rc = launch_rebuild(d, 1, h->st)) return rc;
+    // a replay after regrowth always rebuilds (the regrown lists are empty)
+    if (int rc = launch_rebuild(d, (force_rebuild || attempt > 0) ? 1 : 0, h->st)) return rc;
     if (int rc = h->launch_far(0)) return rc;
     if (int rc = launch_force(d, MODE_FORCES, 0, h->fp32, h->st)) return rc;
     int rc = h->check_and_fix();
@@ -711,6 +840,7 @@ int sf_forces(sf_handle* h, double* f, double* fs, double* fb, double* fe, doubl
     for (auto& x : s) { x.need_lj = x.need_q = x.need_cand = x.deformed = x.need_cell_lj = x.need_cell_q = 0; }
     HIPCHK(hipMemcpyAsync(h->d.rs, s.data(), h->R * sizeof(RepState), hipMemcpyHostToDevice, h->st));
   }
+  if (int rc = launch_lists_done(h->d, h->st)) return rc;
   double* outs[5] = {f, fs, fb, fe, flj};
   for (int k = 0; k < 5; k++)
     if (outs[k]) HIPCHK(hipMemcpyAsync(outs[k], h->out_buf + k * n3, n3 * sizeof(double), hipMemcpyDeviceToHost, h->st));
@@ -751,18 +881,49 @@ int sf_step(sf_handle* h, int n_steps) {
   double ti = 0, trb = 0, tf = 0, tr = 0, nt = 0, tfar = 0;
   int done = 0, rc = SF_OK;
   int retries = 0;
-  while (done < n_steps) {
-    const int n = std::min(kChunk, n_steps - done);
+  int goal = n_steps;
+  // a replica that became non-finite stops the handle at its last finite state
+  bool nonfinite = false;
+  long long nf_step = 0;
+  int nf_rep = -1;
+  while (done < goal) {
+    const int n = std::min(kChunk, goal - done);
     if ((rc = h->snapshot(true))) break;
     if ((rc = h->launch_chunk(n))) break;
     rc = h->check_and_fix();
     if (rc < 0) break;
+    if (rc == 2) {
+      // back to the chunk's start, then forward to the step before the first non-finite one
+      if (nonfinite) { set_error("non-finite state reappeared while replaying to the last finite step"); rc = SF_E_NONFINITE; break; }
+      nonfinite = true;
+      nf_step = h->bad_step;
+      nf_rep = h->bad_rep;
+      if ((rc = h->snapshot(false)) || (rc = h->clear_flags_force_rebuild())) break;
+      std::vector<RepState> st0;
+      if ((rc = h->read_states(st0))) break;
+      const long long k = nf_step - 1 - st0[0].step;
+      if (k < 0 || k >= n) { set_error("non-finite step outside the chunk that reported it"); rc = SF_E_NONFINITE; break; }
+      goal = done + (int)k;
+      rc = SF_OK;
+      continue;
+    }
     if (rc == 1) {
       if (++retries > 16) { set_error("list capacity did not converge"); rc = SF_E_OVERFLOW; break; }
       if ((rc = h->snapshot(false))) break;
       if ((rc = h->clear_flags_force_rebuild())) break;
       rc = SF_OK;
       continue;
+    }
+    if (h->far_model) {  // audit the far-field model after every chunk
+      double err = 0;
+      if ((rc = h->audit(&err))) break;
+      if (!(err <= h->far_tol)) {
+        // the chunk ran on a model that fails the audit: it is run again with the exact far field
+        h->far_model = false;
+        h->far_fell_back = true;
+        if ((rc = h->snapshot(false)) || (rc = h->clear_flags_force_rebuild())) break;
+        continue;
+      }
     }
     if (h->timing) {
       for (int k = 0; k < n; k += kTimeEvery) {
@@ -778,11 +939,6 @@ int sf_step(sf_handle* h, int n_steps) {
       }
     }
     done += n;
-    if (h->far_model) {  // audit the far-field model after every chunk
-      double err = 0;
-      if ((rc = h->audit(&err))) break;
-      if (!(err <= h->far_tol)) { h->far_model = false; h->far_fell_back = true; }
-    }
   }
   if (rc) return rc;
   if ((rc = launch_apply_com(h->d, h->st))) return rc;
@@ -790,6 +946,11 @@ int sf_step(sf_handle* h, int n_steps) {
   if (h->timing && nt > 0) {
     h->t_int = ti / nt; h->t_rb = trb / nt; h->t_force = tf / nt; h->t_red = tr / nt; h->t_steps = nt;
     h->t_far = tfar / nt;
+  }
+  if (nonfinite) {
+    set_error("replica " + std::to_string(nf_rep) + " became non-finite at step " + std::to_string(nf_step) +
+              "; the handle holds the last finite state (step " + std::to_string(nf_step - 1) + ")");
+    return SF_E_NONFINITE;
   }
   return SF_OK;
 }
@@ -970,7 +1131,7 @@ int sf_set_far_model(sf_handle* h, const sf_far_model* m, double tol) {
     FarSpec* ds; unsigned short* dw; float* db; float* fr;
     if (h->alloc(&ds, 1) || h->alloc(&dw, kFPackN) || h->alloc(&db, 3 * kFHid + kFOutPad) ||
         h->alloc(&fr, (size_t)h->Ns * 12) || h->alloc(&h->qfar_exact, 3 * (size_t)std::max(h->Nq, 1)) ||
-        h->alloc(&h->audit_out, 2))
+        h->alloc(&h->audit_out, 3 * (size_t)h->R))
       return SF_E_HIP;
     h->fdev.spec = ds; h->fdev.wpack = dw; h->fdev.bias = db; h->fdev.frames = fr;
   }
@@ -981,7 +1142,7 @@ int sf_set_far_model(sf_handle* h, const sf_far_model* m, double tol) {
   h->far_tol = tol;
   h->far_model = true;
   h->far_fell_back = false;
-  h->far_audits = 0; h->far_last = 0; h->far_max = 0;
+  h->far_audits = 0; h->far_last = 0; h->far_max = 0; h->far_last_force = 0; h->far_max_force = 0;
   return SF_OK;
 }
 
@@ -1004,6 +1165,7 @@ int sf_get_far_stats(sf_handle* h, sf_far_stats* o) {
   if (!h || !o) { set_error("null argument"); return SF_E_ARG; }
   o->audits = h->far_audits; o->last_err = h->far_last; o->max_err = h->far_max;
   o->using_model = h->far_model ? 1 : 0; o->fell_back = h->far_fell_back ? 1 : 0;
+  o->last_err_force = h->far_last_force; o->max_err_force = h->far_max_force;
   return SF_OK;
 }
 

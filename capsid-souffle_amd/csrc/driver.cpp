@@ -48,6 +48,11 @@ struct Opts {
   double far_rf = 0;       // --far-rf: with -e 0, centre distance beyond which pairs are far field
                            // (0: 2.5 template diameters)
   double oligomers = 0;    // --oligomers <contact>: oligomer statistics at movie steps (0: off)
+  int capsid = 20;         // --capsid <n>: subunits of a complete capsid (completion fraction)
+  int digits = 17;         // --restart-digits <n>: significant digits of restart files (the
+                           // reference writes 12, and 6 for the first velocity)
+  int resume = 1;          // --resume <0|1>: continue from a restart's resume_<a>.txt sidecar when
+                           // it exists (bit for bit); 0: restart as the reference does
 };
 
 bool parse_bool(const std::string& s) { return s == "1" || s == "true" || s == "yes" || s == "on"; }
@@ -59,7 +64,8 @@ int parse(int argc, char** argv, Opts& o) {
     if (a == "-h" || a == "--help") {
       std::cout << "Usage:\nsouffle-gpu <options>  (flags of the reference: -D -f -C -c -S -s -b -T -t -r -d -q -Q -K -e\n"
                    "  -R -X -V -E -B -L -M -W -w -N -p -a -G -m -H; GPU only: --skin <sigma>, --fp32 <0|1>,\n"
-                   "  --far-rf <sigma> (with -e 0), --oligomers <contact sigma>)\n";
+                   "  --far-rf <sigma> (with -e 0), --oligomers <contact sigma>, --capsid <subunits>,\n"
+                   "  --restart-digits <n>, --resume <0|1>)\n";
       return 1;
     }
     std::string key, val;
@@ -117,6 +123,9 @@ int parse(int argc, char** argv, Opts& o) {
     else if (k == "fp32") o.fp32 = atoi(v.c_str());
     else if (k == "far-rf") o.far_rf = atof(v.c_str());
     else if (k == "oligomers") o.oligomers = atof(v.c_str());
+    else if (k == "capsid") o.capsid = atoi(v.c_str());
+    else if (k == "restart-digits") o.digits = atoi(v.c_str());
+    else if (k == "resume") o.resume = atoi(v.c_str());
     else { std::cerr << "unknown option -" << k << "\n"; return -1; }
   }
   return 0;
@@ -177,6 +186,51 @@ void progress(double fraction) {  // src/functions.cpp:24-30
 int fail(const char* what) {
   std::cerr << what << ": " << sf_last_error() << "\n";
   return 1;
+}
+
+// restart_<a>.out in the reference's layout (src/md.cpp:526-536): header, then per bead
+// `i vx vy vz x y z`; every value at `digits` significant digits (17 round-trips a double)
+void write_restart(const std::string& path, long long a, const std::vector<double>& pos, const std::vector<double>& vel,
+                   long long N, int digits) {
+  std::ofstream out(path.c_str(), std::ios::out);
+  out << "Velocities & Positions for " << a << std::endl;
+  out << std::setprecision(digits);
+  for (long long i = 0; i < N; i++)
+    out << i << "  " << vel[3 * i] << std::setw(25) << vel[3 * i + 1] << std::setw(25) << vel[3 * i + 2] << std::setw(25)
+        << pos[3 * i] << std::setw(25) << pos[3 * i + 1] << std::setw(25) << pos[3 * i + 2] << std::setw(25) << std::endl;
+}
+
+// Sidecar of a restart: what the reference does not keep across a restart (the chain, the BD
+// engine, the step counter and, for BD, the last integrator force) at full precision, so that a
+// resumed run continues bit for bit.
+void write_sidecar(const std::string& path, long long a, const sf_resume& r, const std::vector<double>* tforce) {
+  std::ofstream out(path.c_str(), std::ios::out);
+  out << std::setprecision(17);
+  out << "iteration " << a << "\nstep " << r.step << "\nq_chain " << r.q_chain << "\nrng " << r.rng << "\nke " << r.ke
+      << "\nxi";
+  for (int j = 0; j < r.q_chain; j++) out << " " << r.xi[j];
+  out << "\neta";
+  for (int j = 0; j < r.q_chain; j++) out << " " << r.eta[j];
+  out << "\ntforce " << (tforce ? tforce->size() / 3 : 0) << "\n";
+  if (tforce)
+    for (size_t i = 0; i < tforce->size(); i += 3) out << (*tforce)[i] << " " << (*tforce)[i + 1] << " " << (*tforce)[i + 2] << "\n";
+}
+
+bool read_sidecar(const std::string& path, long long& a, sf_resume& r, std::vector<double>& tforce) {
+  std::ifstream in(path.c_str());
+  if (!in) return false;
+  std::string key;
+  std::memset(&r, 0, sizeof(r));
+  long long nt = 0;
+  in >> key >> a >> key >> r.step >> key >> r.q_chain >> key >> r.rng >> key >> r.ke >> key;
+  if (!in || r.q_chain < 0 || r.q_chain > 8) return false;
+  for (int j = 0; j < r.q_chain; j++) in >> r.xi[j];
+  in >> key;
+  for (int j = 0; j < r.q_chain; j++) in >> r.eta[j];
+  in >> key >> nt;
+  tforce.assign(3 * (size_t)nt, 0.0);
+  for (size_t i = 0; i < tforce.size(); i++) in >> tforce[i];
+  return (bool)in;
 }
 
 }  // namespace
@@ -256,6 +310,9 @@ int main(int argc, char** argv) {
   // initial state: restart file or the reference's lattice and velocities
   std::vector<double> pos(3 * N), vel(3 * N);
   long long loopStart = 0;
+  sf_resume resume_rec;
+  std::vector<double> resume_force;
+  bool have_resume = false;
   bool restart = o.R;
   std::string rname = restart ? newest_restart() : "";
   if (rname.empty()) {
@@ -270,6 +327,18 @@ int main(int argc, char** argv) {
     rin >> d >> d >> d >> d >> loopStart;
     for (long long i = 0; i < N; i++) rin >> d >> vel[3 * i] >> vel[3 * i + 1] >> vel[3 * i + 2] >> pos[3 * i] >> pos[3 * i + 1] >> pos[3 * i + 2];
     if (!rin) { std::cerr << "restart file has fewer beads than the system\n"; return 1; }
+    // a sidecar written with this restart continues the run bit for bit from the next iteration
+    if (o.resume) {
+      std::ostringstream sn;
+      sn << "outfiles/resume_" << loopStart << ".txt";
+      long long a_side = -1;
+      if (read_sidecar(sn.str(), a_side, resume_rec, resume_force)) {
+        if (a_side != loopStart) { std::cerr << sn.str() << " belongs to another iteration\n"; return 1; }
+        if (brownian && resume_force.size() != 3 * (size_t)N) { std::cerr << sn.str() << " lacks the BD force\n"; return 1; }
+        have_resume = true;
+        std::cout << "Resuming bit for bit from " << sn.str() << std::endl;
+      }
+    }
   } else {
     const long double bx = prm.box;
     const int nf = (int)std::ceil(std::pow((double)ns, 1.0 / 3.0));
@@ -349,7 +418,27 @@ int main(int argc, char** argv) {
     }
     if (sf_set_big_beads(h, nbig, big_pos.data(), big_sigma, o.G, o.H)) return fail("sf_set_big_beads");
   }
-  if (sf_set_state(h, pos.data(), vel.data())) return fail("sf_set_state");
+  if (have_resume) {
+    if (sf_set_state_resume(h, pos.data(), vel.data(), &resume_rec, resume_force.empty() ? nullptr : resume_force.data()))
+      return fail("sf_set_state_resume");
+    loopStart += 1;  // the restart's own iteration is done
+  } else if (sf_set_state(h, pos.data(), vel.data())) {
+    return fail("sf_set_state");
+  }
+  long long lib_step = have_resume ? resume_rec.step : 0;  // the library's loop-iteration counter
+  // restart and sidecar of the current state (iteration a done)
+  auto checkpoint = [&](const std::string& stem, long long a) -> int {
+    if (sf_get_state(h, pos.data(), vel.data())) return fail("sf_get_state");
+    sf_resume rec;
+    std::vector<double> tf(brownian ? 3 * (size_t)N : 0);
+    if (sf_get_resume(h, &rec, brownian ? tf.data() : nullptr)) return fail("sf_get_resume");
+    std::ostringstream nm, sn;
+    nm << "outfiles/" << stem << "_" << a << ".out";
+    sn << "outfiles/" << (stem == "restart" ? "resume" : stem + "_resume") << "_" << a << ".txt";
+    write_restart(nm.str(), a, pos, vel, N, o.digits);
+    write_sidecar(sn.str(), a, rec, brownian ? &tf : nullptr);
+    return 0;
+  };
   std::vector<double> energies(7), per_bead(5 * N);
   double vcom = 0;
   auto energy_row = [&](long long a, bool with_vcom) -> int {
@@ -378,43 +467,56 @@ int main(int argc, char** argv) {
   std::vector<int> olig_label(ns), olig_ms(ns);
   std::vector<double> ms_weighted(ns, 0.0);
   long long olig_frames = 0;
+  // completion fraction per movie frame in the layout of the reference's growth-kinetics table
+  // (data/simulation/cluster.dat: label, time in ns, completion fraction, spread): here the
+  // label is -b, the fraction (mean oligomer size in subunits - 1) / (capsid size - 1), which
+  // is 0 for free subunits and 1 when every subunit sits in an oligomer of capsid size, and the
+  // spread nan (one run has none)
+  std::ofstream cluster_out;
   if (o.oligomers > 0) {
+    if (o.capsid < 2) { std::cerr << "--capsid must be at least 2\n"; return 1; }
     olig_out.open("outfiles/oligomers.out", std::ios_base::app);
     olig_out << "time" << std::setw(15) << "oligomers" << std::setw(15) << "largest" << std::setw(15)
              << "mass_spectrum(1.." << ns << ")" << std::endl;
+    cluster_out.open("outfiles/cluster.dat", std::ios_base::app);
+    cluster_out << "# label  time(ns)  cluster_avg  std" << std::endl;
   }
 
   const double nsteps_d = totaltime / o.dt;
   long long a = loopStart;
   double percentagePre = -1;
-  std::ofstream restart_out, rb_out;
+  std::ofstream rb_out;
   while ((double)a < nsteps_d + 1) {
     // run up to the next iteration that writes output (or the last one)
     long long stop = a;
     while ((double)(stop + 1) < nsteps_d + 1 && stop % o.W != 0 && stop % o.w != 0 && stop % o.M != 0) stop++;
-    if (sf_step(h, (int)(stop - a + 1))) return fail("sf_step");
+    const int rc_step = sf_step(h, (int)(stop - a + 1));
+    if (rc_step == SF_E_NONFINITE) {
+      // the handle holds the last finite state, after iteration a + k - 1 where k is how far the
+      // library's step counter got in this call: kept for inspection under a name without
+      // "restart", so that a rerun does not resume from it
+      std::cerr << "\nsf_step: " << sf_last_error() << "\n";
+      sf_resume rec;
+      if (sf_get_resume(h, &rec, nullptr)) return fail("sf_get_resume");
+      const long long a_last = a + (rec.step - lib_step) - 1;
+      checkpoint("lastfinite", a_last);
+      std::cerr << "last finite state (iteration " << a_last << ") written to outfiles/lastfinite_" << a_last << ".out\n";
+      return 1;
+    }
+    if (rc_step) return fail("sf_step");
+    lib_step += stop - a + 1;
     a = stop;
-    if (a % o.w == 0) {  // src/md.cpp:526-536
-      sf_get_state(h, pos.data(), vel.data());
-      std::ostringstream nm;
-      nm << "outfiles/restart_" << a << ".out";
-      restart_out.open(nm.str().c_str(), std::ios::out);
-      restart_out << "Velocities & Positions for " << a << std::endl;
-      for (long long i = 0; i < N; i++)
-        restart_out << i << "  " << vel[3 * i] << std::setw(25) << std::setprecision(12) << vel[3 * i + 1] << std::setw(25)
-                    << std::setprecision(12) << vel[3 * i + 2] << std::setw(25) << std::setprecision(12) << pos[3 * i]
-                    << std::setw(25) << std::setprecision(12) << pos[3 * i + 1] << std::setw(25) << std::setprecision(12)
-                    << pos[3 * i + 2] << std::setw(25) << std::setprecision(12) << std::endl;
-      restart_out.close();
+    if (a % o.w == 0) {  // src/md.cpp:526-536, with the sidecar of this build
+      if (checkpoint("restart", a)) return 1;
       if (nbig > 0) {  // src/md.cpp:537-545
         std::ostringstream rn;
         rn << "outfiles/rb_" << a << ".out";
         rb_out.open(rn.str().c_str(), std::ios::out);
+        rb_out << std::setprecision(o.digits);
         for (int i = 0; i < nbig; i++)
-          rb_out << big_vel[3 * i] << std::setw(25) << std::setprecision(12) << big_vel[3 * i + 1] << std::setw(25)
-                 << std::setprecision(12) << big_vel[3 * i + 2] << std::setw(25) << std::setprecision(12) << big_pos[3 * i]
-                 << std::setw(25) << std::setprecision(12) << big_pos[3 * i + 1] << std::setw(25) << std::setprecision(12)
-                 << big_pos[3 * i + 2] << std::setw(25) << std::setprecision(12) << std::endl;
+          rb_out << big_vel[3 * i] << std::setw(25) << big_vel[3 * i + 1] << std::setw(25) << big_vel[3 * i + 2]
+                 << std::setw(25) << big_pos[3 * i] << std::setw(25) << big_pos[3 * i + 1] << std::setw(25)
+                 << big_pos[3 * i + 2] << std::setw(25) << std::endl;
         rb_out.close();
       }
     }
@@ -437,6 +539,9 @@ int main(int argc, char** argv) {
       }
       olig_out << std::endl;
       olig_frames++;
+      const double frac = ((double)ns / n - 1.0) / (o.capsid - 1.0);
+      cluster_out << o.kb << "  " << std::fixed << std::setprecision(6) << a * o.dt * SItime / 1e-9 << "  " << frac
+                  << "  nan" << std::defaultfloat << std::endl;
     }
     if (a % o.M == 0) {  // src/md.cpp:647-669
       sf_get_state(h, pos.data(), nullptr);

@@ -203,6 +203,35 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
   }
 }
 
+// Positions set from outside the step loop (sf_set_positions): centre beads and charged-bead
+// copies refreshed, the replica's list build requested when a bead moved more than skin/2 from
+// its position at the last build (the test of k_integrate), cell counts cleared for that build.
+__global__ __launch_bounds__(256) void k_moved(Dev d) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= d.Nb) return;
+  const int np = d.topo.np;
+  const int s = b / np, l = b - s * np;
+  const int r = rep_of(d, s);
+  const RepParams& P = d.rp[r];
+  const double4 p = d.pos[b], q = d.xb[b];
+  const double ddx = min_img(p.x - q.x, P.box, P.hbox), ddy = min_img(p.y - q.y, P.box, P.hbox),
+               ddz = min_img(p.z - q.z, P.box, P.hbox);
+  if (!(ddx * ddx + ddy * ddy + ddz * ddz <= P.half_skin2)) d.rs[r].rebuild = 1;  // same value from any thread
+  if (l == d.topo.centre) d.centres[s] = p;
+  const int qs = d.topo.qslot[l];
+  if (qs >= 0) d.qpos[(size_t)s * d.topo.nq + qs] = p;
+  if (d.cells) {
+    for (int c = b; c < d.ncell_lj; c += d.Nb) d.cnt_lj[c] = 0;
+    for (int c = b; c < d.ncell_q; c += d.Nb) d.cnt_q[c] = 0;
+  }
+}
+
+// Lists are current after a force evaluation outside the step loop: no build pending.
+__global__ void k_lists_done(Dev d) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < d.R) d.rs[r].rebuild = 0;
+}
+
 // ------------------------------------------------------------------------------------------
 // Neighbour lists.
 // LJ list of bead i: beads j of other subunits within (cutoff of the pair + skin), the cutoff
@@ -1620,8 +1649,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     part[0] = 0.5 * m * (vx * vx + vy * vy + vz * vz);
     part[1] = vx; part[2] = vy; part[3] = vz;
     if (P.brownian) {
-      // noise of the next step (or of the first step at a start) for its mean
-      const uint32_t nxt = init ? P.seed : lcg_mulmod(S.rng, P.mult6n);
+      // noise of the next step (or, at a start, of the first step: the engine state the start was
+      // given, the seed for a fresh start) for its mean
+      const uint32_t nxt = init ? S.rng : lcg_mulmod(S.rng, P.mult6n);
       double ux, uy, uz;
       bd_draws(nxt, gi - P.b0, ux, uy, uz);
       const double amp = P.bd_amp[tid];
@@ -1655,15 +1685,19 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
   // chain constants and state to LDS (lane j: bath j), the scalars to registers
   const int q = P.q_chain, brownian = P.brownian;
   const bool lists = init || S.rebuild;
+  // a (re)start continues from the chain state in RepState (zero for a fresh start) and, when
+  // resuming, from the kinetic energy the interrupted run's chain would have used
+  const int resume = init ? S.resume : 0;
+  const double ke_saved = S.ke;
   double xi_l = 0.0;
   if (tid < kMaxChain) {
     cQ[tid] = P.Qb[tid]; cinvQ[tid] = P.inv_Qb[tid]; cdof[tid] = P.dof[tid];
-    const double xn = init ? 0.0 : S.xi_n[tid];
-    cxi[tid] = xn; ceta[tid] = init ? 0.0 : S.eta_n[tid];
+    const double xn = init ? S.xi[tid] : S.xi_n[tid];
+    cxi[tid] = xn; ceta[tid] = init ? S.eta[tid] : S.eta_n[tid];
     if (!init && !brownian) xi_l = xn;
   }
   const double dt = P.dt, T = P.T, inv_nb = P.inv_nb;
-  const uint32_t seed = P.seed, mult6n = P.mult6n, rng0 = S.rng;
+  const uint32_t mult6n = P.mult6n, rng0 = S.rng;
   const long long step0 = S.step, rebuilds0 = S.rebuilds;
   // slot-major partials: a wave reads 64 consecutive subunits of one slot per load; each thread
   // adds its subunits tid, tid + kFinThreads, ... in that order, with the loads of kFinBatch of
@@ -1705,7 +1739,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
   }
   __syncthreads();
   if (tid >= 64) return;
-  const double ke = tot[0];
+  const double ke = resume ? ke_saved : tot[0];
   if (!brownian) {
     // forward chain of this step (src/functions.cpp:33-58 in bath order): bath j's update
     // reads xi[j+1] before that bath is updated, so lane j + 1 evaluates both exponentials of
@@ -1758,6 +1792,13 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
   }
   if (tid != 0) return;
   S.ke = ke;
+  // non-finite guard: the kinetic energy sums every velocity, so one non-finite bead shows here
+  const double vsum = tot[0] + tot[1] + tot[2] + tot[3];
+  if (!isfinite(vsum) && !S.nonfinite) {
+    S.nonfinite = 1;
+    S.bad_step = init ? step0 : step0 + 1;
+  }
+  S.resume = 0;
   if (init) {
     S.mean_v[0] = S.mean_v[1] = S.mean_v[2] = 0.0;
     S.vcom = 0.0;
@@ -1768,7 +1809,7 @@ __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
     S.step = step0 + 1;
   }
   if (brownian) {
-    S.rng = init ? seed : lcg_mulmod(rng0, mult6n);
+    S.rng = init ? rng0 : lcg_mulmod(rng0, mult6n);  // a (re)start keeps the engine state it was given
     S.mean_fran[0] = tot[4] * inv_nb; S.mean_fran[1] = tot[5] * inv_nb; S.mean_fran[2] = tot[6] * inv_nb;
   }
   if (lists) {
@@ -2010,6 +2051,14 @@ int launch_integrate(const Dev& d, int mode, hipStream_t st) {
   return check_launch();
 }
 
+int launch_moved(const Dev& d, hipStream_t st) {
+  hipLaunchKernelGGL(k_moved, dim3((d.Nb + 255) / 256), dim3(256), 0, st, d);
+  return check_launch();
+}
+int launch_lists_done(const Dev& d, hipStream_t st) {
+  hipLaunchKernelGGL(k_lists_done, dim3((d.R + 63) / 64), dim3(64), 0, st, d);
+  return check_launch();
+}
 int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st) {
   if (!d.cells) {
     hipLaunchKernelGGL(k_rebuild, dim3(d.Ns), dim3(kBlock), 0, st, d, force_rebuild);

@@ -92,6 +92,8 @@ int launch_pack_weights(const float* params, unsigned short* wpack, float* bias,
 int launch_mlp_rows(const unsigned short* wpack, const float* bias, int n, const float* x, float* y, hipStream_t st);
 int launch_frames(const Dev& d, const FarDev& f, hipStream_t st);
 int launch_far_mlp(const Dev& d, const FarDev& f, hipStream_t st);
-int launch_far_audit(const Dev& d, const double* qfar_model, double* out2, hipStream_t st);
+// per replica r: out2[3 r] = sum |F_far,model - F_far,exact|^2, out2[3 r + 1] = sum |F_far,exact|^2,
+// out2[3 r + 2] = sum over its beads of |F|^2 with the exact far field
+int launch_far_audit(const Dev& d, const double* qfar_exact, double* out2, hipStream_t st);
 
 }  // namespace sf

@@ -151,6 +151,9 @@ struct RepState {
   int need_cand;        // candidate subunits when they exceeded kMaxCand
   int deformed;         // a bead left the centre-bead bound at a list build
   int need_cell_lj, need_cell_q;  // largest cell occupancy seen when it exceeded the capacity
+  int nonfinite;        // the kinetic energy or the centre-of-mass velocity became non-finite
+  int resume;           // a (re)start takes ke from this state instead of measuring it
+  long long bad_step;   // step counter value of the first non-finite step
   long long step, rebuilds, lj_entries, q_entries;
 };
 
@@ -214,6 +217,8 @@ constexpr int kOligoScratchSubunits = 1 << 18;
 int launch_centres(const Dev& d, hipStream_t st);
 int launch_integrate(const Dev& d, int mode, hipStream_t st);
 int launch_rebuild(const Dev& d, int force_rebuild, hipStream_t st);
+int launch_moved(const Dev& d, hipStream_t st);       // after positions set from outside the loop
+int launch_lists_done(const Dev& d, hipStream_t st);  // clears the pending-build flags
 int launch_force(const Dev& d, int mode, int init, int fp32, hipStream_t st);  // uses d.tile
 bool tile_supported(int tile);
 int launch_finalize(const Dev& d, int init, hipStream_t st);

@@ -715,22 +715,46 @@ __global__ __launch_bounds__(kMlpThreads) void k_far_mlp(Dev d, FarDev f) {
 
 // Audit sums over all beads: out[0] = sum over charged beads |qfar_model - qfar_exact|^2,
 // out[1] = sum over beads |F|^2 (the integrator force after the step's force kernel).
+// Audit sums of one replica per workgroup, in a fixed order: out[3 r] = sum over its charged beads
+// of |F_far,model - F_far,exact|^2, out[3 r + 1] = sum of |F_far,exact|^2 (the far field the model
+// replaces), out[3 r + 2] = sum over all its beads of |F|^2 with the exact far field, F being the
+// integrator force of the last step (MD: the conservative force) with the model's far field
+// swapped for the exact one.
 __global__ __launch_bounds__(1024) void k_far_audit(Dev d, const double* qfar_exact, double* out) {
-  __shared__ double red[2][16];
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < d.Nq * 3; i += 1024) {
-    const double e = d.qfar[i] - qfar_exact[i];
+  __shared__ double red[3][16];
+  const RepParams& P = d.rp[blockIdx.x];
+  const int np = d.topo.np, nq = d.topo.nq;
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = 3 * P.q0 + threadIdx.x; i < 3 * (P.q0 + P.nqr); i += 1024) {
+    const double x = qfar_exact[i], e = d.qfar[i] - x;
     a += e * e;
+    b += x * x;
   }
-  for (int i = threadIdx.x; i < d.Nb; i += 1024) b += d.fx[i] * d.fx[i] + d.fy[i] * d.fy[i] + d.fz[i] * d.fz[i];
+  for (int i = P.b0 + threadIdx.x; i < P.b0 + P.nb; i += 1024) {
+    double fx = d.fx[i], fy = d.fy[i], fz = d.fz[i];
+    const int l = i % np, qs = d.topo.qslot[l];
+    if (qs >= 0) {
+      const size_t k = 3 * ((size_t)(i / np) * nq + qs);
+      fx += qfar_exact[k] - d.qfar[k];
+      fy += qfar_exact[k + 1] - d.qfar[k + 1];
+      fz += qfar_exact[k + 2] - d.qfar[k + 2];
+    }
+    c += fx * fx + fy * fy + fz * fz;
+  }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) { a += __shfl_down(a, o, 64); b += __shfl_down(b, o, 64); }
-  if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = a; red[1][threadIdx.x >> 6] = b; }
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+    c += __shfl_down(c, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a; red[1][threadIdx.x >> 6] = b; red[2][threadIdx.x >> 6] = c;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double sa = 0, sb = 0;
-    for (int w = 0; w < 16; w++) { sa += red[0][w]; sb += red[1][w]; }
-    out[0] = sa; out[1] = sb;
+    double sa = 0, sb = 0, sc = 0;
+    for (int w = 0; w < 16; w++) { sa += red[0][w]; sb += red[1][w]; sc += red[2][w]; }
+    out[3 * blockIdx.x] = sa; out[3 * blockIdx.x + 1] = sb; out[3 * blockIdx.x + 2] = sc;
   }
 }
 
@@ -831,7 +855,7 @@ int launch_far_mlp(const Dev& d, const FarDev& f, hipStream_t st) {
 }
 
 int launch_far_audit(const Dev& d, const double* qfar_exact, double* out2, hipStream_t st) {
-  hipLaunchKernelGGL(k_far_audit, dim3(1), dim3(1024), 0, st, d, qfar_exact, out2);
+  hipLaunchKernelGGL(k_far_audit, dim3(d.R), dim3(1024), 0, st, d, qfar_exact, out2);
   return check_launch_far();
 }
 

@@ -41,6 +41,11 @@ class SfStats(ctypes.Structure):
                 ("k_timed_steps", ctypes.c_double), ("vcom", ctypes.c_double), ("k_far_ms", ctypes.c_double)]
 
 
+class SfResume(ctypes.Structure):
+    _fields_ = [("step", ctypes.c_int64), ("q_chain", ctypes.c_int32), ("rng", ctypes.c_uint32),
+                ("ke", ctypes.c_double), ("xi", ctypes.c_double * 8), ("eta", ctypes.c_double * 8)]
+
+
 class SfTuneOpts(ctypes.Structure):
     _fields_ = [("trial_steps", ctypes.c_int), ("n_skins", ctypes.c_int), ("skins", ctypes.POINTER(ctypes.c_double)),
                 ("n_tiles", ctypes.c_int), ("tiles", ctypes.POINTER(ctypes.c_int))]
@@ -65,7 +70,8 @@ class SfTrainReport(ctypes.Structure):
 
 class SfFarStats(ctypes.Structure):
     _fields_ = [("audits", ctypes.c_int64), ("last_err", ctypes.c_double), ("max_err", ctypes.c_double),
-                ("using_model", ctypes.c_int), ("fell_back", ctypes.c_int)]
+                ("using_model", ctypes.c_int), ("fell_back", ctypes.c_int), ("last_err_force", ctypes.c_double),
+                ("max_err_force", ctypes.c_double)]
 
 
 # every entry point of include/souffle_gpu.h: name -> (restype, argtypes)
@@ -86,6 +92,10 @@ SIGNATURES = {
     "sf_set_state": (ctypes.c_int, [_P, _D, _D]),
     "sf_get_state": (ctypes.c_int, [_P, _D, _D]),
     "sf_get_thermostat": (ctypes.c_int, [_P, ctypes.c_int, _D, _D]),
+    "sf_get_resume": (ctypes.c_int, [_P, ctypes.POINTER(SfResume), _D]),
+    "sf_set_state_resume": (ctypes.c_int, [_P, _D, _D, ctypes.POINTER(SfResume), _D]),
+    "sf_set_positions": (ctypes.c_int, [_P, _D]),
+    "sf_forces_ex": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D, _D, _D]),
     "sf_forces": (ctypes.c_int, [_P, _D, _D, _D, _D, _D]),
     "sf_energies": (ctypes.c_int, [_P, _D, _D]),
     "sf_step": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -138,12 +148,17 @@ def load(path=LIB_PATH):
 
 
 class SfError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+SF_E_NONFINITE = -5
 
 
 def _err(rc, what):
     if rc < 0:
-        raise SfError(f"{what} failed ({rc}): {_lib.sf_last_error().decode()}")
+        raise SfError(f"{what} failed ({rc}): {_lib.sf_last_error().decode()}", rc)
     return rc
 
 
@@ -237,6 +252,41 @@ class System:
         vel = np.zeros((self.n_beads, 3))
         _err(_lib.sf_get_state(self.ptr, _dp(pos), _dp(vel)), "sf_get_state")
         return pos, vel
+
+    def get_resume(self, with_force=True):
+        """(resume records [R] as dicts, integrator force [N,3] or None): what sf_set_state_resume
+        needs besides positions and velocities to continue bit for bit."""
+        recs = (SfResume * self.R)()
+        f = np.zeros((self.n_beads, 3)) if with_force else None
+        _err(_lib.sf_get_resume(self.ptr, recs, _dp(f)), "sf_get_resume")
+        out = []
+        for r in recs:
+            q = r.q_chain
+            out.append(dict(step=r.step, q_chain=q, rng=r.rng, ke=r.ke, xi=list(r.xi)[:q], eta=list(r.eta)[:q]))
+        return out, f
+
+    def set_state_resume(self, pos, vel, records, tforce=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        assert pos.shape == (self.n_beads, 3) and vel.shape == (self.n_beads, 3) and len(records) == self.R
+        recs = (SfResume * self.R)()
+        for r, d in zip(recs, records):
+            r.step, r.q_chain, r.rng, r.ke = int(d["step"]), int(d["q_chain"]), int(d["rng"]), float(d["ke"])
+            for j, (x, e) in enumerate(zip(d["xi"], d["eta"])):
+                r.xi[j], r.eta[j] = x, e
+        tf = None if tforce is None else np.ascontiguousarray(tforce, dtype=np.float64)
+        _err(_lib.sf_set_state_resume(self.ptr, _dp(pos), _dp(vel), recs, _dp(tf)), "sf_set_state_resume")
+
+    def set_positions(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        assert pos.shape == (self.n_beads, 3)
+        _err(_lib.sf_set_positions(self.ptr, _dp(pos)), "sf_set_positions")
+
+    def forces_ex(self, force_rebuild):
+        """sf_forces_ex: (total, stretch, bend, yukawa, lj) with the reference's updatePairlist flag."""
+        out = [np.zeros((self.n_beads, 3)) for _ in range(5)]
+        _err(_lib.sf_forces_ex(self.ptr, 1 if force_rebuild else 0, *[_dp(a) for a in out]), "sf_forces_ex")
+        return out
 
     def thermostat(self, replica=0):
         xi = np.zeros(16)

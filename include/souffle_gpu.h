@@ -32,6 +32,7 @@ extern "C" {
 #define SF_E_HIP (-2)       /* HIP runtime error (no device, out of memory, launch failure) */
 #define SF_E_IO (-3)        /* template or file could not be read */
 #define SF_E_OVERFLOW (-4)  /* a neighbour list outgrew its capacity and could not be regrown */
+#define SF_E_NONFINITE (-5) /* a replica's velocities became non-finite (sf_step stops before it) */
 
 typedef struct sf_template sf_template; /* parsed capsomere template + derived topology */
 typedef struct sf_handle sf_handle;     /* device-resident system or batch of replicas */
@@ -149,6 +150,40 @@ int sf_set_state(sf_handle* h, const double* pos, const double* vel);
 /* Downloads positions and velocities (either may be NULL). */
 int sf_get_state(sf_handle* h, double* pos, double* vel);
 
+/* What a run needs besides positions and velocities to continue bit for bit: the reference keeps
+ * none of it across a restart (src/md.cpp:187-196 starts the chain at zero, src/md.cpp:382 seeds
+ * the BD engine afresh, src/initialize.cpp:166-191 reads positions and velocities only), so a
+ * reference restart is sf_set_state; a bit-continuous resume is sf_set_state_resume. */
+typedef struct {
+  int64_t step;     /* loop iterations done since the start (the reference's counter a) */
+  int32_t q_chain;  /* Nose-Hoover chain length (0 for BD) */
+  uint32_t rng;     /* BD engine state before the next step's draws (minstd_rand0 state) */
+  double ke;        /* kinetic energy the chain's next update uses (measured before the
+                       centre-of-mass removal of the last step, src/md.cpp:456-514) */
+  double xi[8];     /* chain variables after the last completed step (src/thermostat.h:8-54) */
+  double eta[8];
+} sf_resume;
+
+/* Resume records of every replica (out[n_replicas]) and, when tforce is not NULL, the integrator
+ * force [N][3] the next step's first half kick uses (for BD it holds the last step's noise and
+ * drag, src/md.cpp:462-498, so a BD resume needs it; for MD it equals the force at the current
+ * positions). */
+int sf_get_resume(sf_handle* h, sf_resume* out, double* tforce);
+
+/* As sf_set_state, but every replica continues from its resume record (step counter, chain,
+ * kinetic energy, BD engine) instead of starting afresh, and, when given, with the integrator
+ * force tforce [N][3] (required for BD). With the positions, velocities, records and force of
+ * sf_get_state / sf_get_resume, the steps that follow are bit for bit those of the uninterrupted
+ * run. */
+int sf_set_state_resume(sf_handle* h, const double* pos, const double* vel, const sf_resume* res,
+                        const double* tforce);
+
+/* Replaces positions only, for callers that run their own integrator and call the engine where
+ * the reference calls forceCalculation (src/md.cpp:241,443): velocities, thermostat and BD
+ * engine are kept, and a replica's lists are marked for a build when any of its beads moved more
+ * than skin/2 since the last one. */
+int sf_set_positions(sf_handle* h, const double* pos);
+
 /* Nose-Hoover chain variables of one replica (arrays of q_chain); returns q_chain. */
 int sf_get_thermostat(sf_handle* h, int replica, double* xi, double* eta);
 
@@ -158,12 +193,21 @@ int sf_get_thermostat(sf_handle* h, int replica, double* xi, double* eta);
 int sf_forces(sf_handle* h, double* f_total, double* f_stretch, double* f_bend,
               double* f_yukawa, double* f_lj);
 
+/* sf_forces with the reference's updatePairlist argument (src/forces.h:13-15): force_rebuild = 1
+ * builds the lists anew; 0 builds them only if a bead moved more than skin/2 since the last
+ * build (the forces are the same either way: the lists hold every pair inside the cutoffs). */
+int sf_forces_ex(sf_handle* h, int force_rebuild, double* f_total, double* f_stretch, double* f_bend,
+                 double* f_yukawa, double* f_lj);
+
 /* Energies at the current state (src/md.cpp:553-605): totals[n_replicas][7] =
  * {kinetic, stretching, bending, LJ, Yukawa, thermostat potential, thermostat kinetic};
  * per_bead (nullable) [N][5] = {se, be, ne, ce, ke} as the reference's BEAD fields. */
 int sf_energies(sf_handle* h, double* totals, double* per_bead);
 
-/* Runs n_steps iterations of the step loop body (src/md.cpp:400-514) for every replica. */
+/* Runs n_steps iterations of the step loop body (src/md.cpp:400-514) for every replica.
+ * If a replica's velocities become non-finite (the reference carries on with NaN), every replica
+ * is left at the last finite step and SF_E_NONFINITE is returned; sf_last_error names the step
+ * and the replica, and sf_get_state / sf_get_resume give that state. */
 int sf_step(sf_handle* h, int n_steps);
 
 /* Per-kernel timing with HIP events on the library's stream during sf_step (off by default). */
@@ -271,23 +315,28 @@ int sf_far_model_train(sf_far_model* m, const sf_train_opts* opts, sf_train_repo
 
 /* Uses model m for the far field of an uncut-Yukawa handle (NULL: exact far field again).
  * The handle copies what it needs; m may be destroyed afterwards. After every 100-step chunk
- * of sf_step the far field is audited against the exact one: if
- * |F_far,model - F_far,exact| / |F| over all beads exceeds tol, the handle falls back to the
- * exact far field for the rest of its life (sf_far_stats reports it). MD only. */
+ * of sf_step the far field is audited against the exact one, per replica relative to the far
+ * field the model replaces: if sqrt(sum |F_far,model - F_far,exact|^2 / sum |F_far,exact|^2)
+ * over a replica's charged beads exceeds tol, that chunk is run again from its start with the
+ * exact far field, which the handle keeps for the rest of its life (sf_far_stats reports it).
+ * MD only. */
 int sf_set_far_model(sf_handle* h, const sf_far_model* m, double tol);
 
 /* drop = 1: leave the far field out (the uncut Yukawa truncated at far_rf by centre-bead
  * distance), the method without a far-field model; 0: back to the exact or model far field. */
 int sf_far_drop(sf_handle* h, int drop);
 
-/* Audit now: relative error of the model's far field against the exact one, as above. */
+/* Audit now: relative error of the model's far field against the exact one, as above (the
+ * largest over the replicas). */
 int sf_far_audit(sf_handle* h, double* rel_err);
 
 typedef struct {
   int64_t audits;
-  double last_err, max_err;
+  double last_err, max_err;  /* audit error relative to the far field (the criterion) */
   int using_model;       /* 1 while the model supplies the far field */
   int fell_back;         /* 1 if an audit exceeded the tolerance */
+  double last_err_force, max_err_force;  /* the same difference relative to the total force,
+                            sqrt(sum |F_far,model - F_far,exact|^2 / sum |F|^2) over a replica */
 } sf_far_stats;
 
 int sf_get_far_stats(sf_handle* h, sf_far_stats* out);

@@ -45,10 +45,19 @@ def test_driver_outputs_and_resume(tmp_path):
     for a in (0, 50, 100):
         assert (out / f"restart_{a}.out").exists()
     assert open(out / "ovito.lammpstrj").read().count("ITEM: TIMESTEP") == 3
-    # resume from the newest restart file (restart_100.out) and run to a = 150
+    assert (out / "resume_100.txt").exists()
+    # the reference's restart semantics (--resume 0): from the newest restart file, iteration 100
+    # again, thermostat and BD engine afresh, run to a = 150
+    ref_dir = tmp_path / "refstyle"
+    shutil.copytree(tmp_path / "infiles", ref_dir / "infiles")
+    shutil.copytree(out, ref_dir / "outfiles")
+    _run(ref_dir, ["-T", "150", "-W", "50", "-w", "50", "-M", "1000", "-R", "true", "--resume", "0"])
+    rows_ref = _rows(ref_dir / "outfiles" / "energy.out")
+    assert [r[0] for r in rows_ref[len(rows):]] == [0, 100, 150]
+    # default: the sidecar continues the run from iteration 101
     _run(tmp_path, ["-T", "150", "-W", "50", "-w", "50", "-M", "1000", "-R", "true"])
     rows2 = _rows(out / "energy.out")
-    assert [r[0] for r in rows2[len(rows):]] == [0, 100, 150]
+    assert [r[0] for r in rows2[len(rows):]] == [0, 150]
     assert (out / "restart_150.out").exists()
 
 
@@ -102,8 +111,115 @@ def test_driver_encapsulation(tmp_path):
     assert text.count("ITEM: NUMBER OF ATOMS\n%d\n" % (n_sub + 2)) == 3
     last = text.strip().splitlines()[-1].split()
     assert int(last[0]) == n_sub + 2 and int(last[1]) == 3 and float(last[6]) == -78
-    # resume: subunits from restart_100.out, big beads from rb_100.out
+    # resume: subunits from restart_100.out (and its sidecar: iteration 101 on), big beads from
+    # rb_100.out
     _run(tmp_path, ["-T", "150", "-W", "50", "-w", "50", "-M", "1000", "-R", "true", "-p", "true", "-m", "2"])
     rows2 = _rows(out / "capenergy.out")
-    assert [r[0] for r in rows2] == [0, 100, 150]
+    assert [r[0] for r in rows2] == [0, 150]
     assert np.array_equal(np.loadtxt(out / "rb_150.out"), big[0])
+
+
+@pytest.mark.parametrize("mode", ["m", "b"])
+def test_driver_resume_bit_for_bit(tmp_path, mode):
+    """200 iterations in one run and 100 + resume + 100 write the same restart_200.out (17 digits
+    round-trip every double) and the same last energy row."""
+    flags = ["-D", mode, "-W", "100", "-w", "100", "-M", "1000"]
+    for d in ("one", "two"):
+        os.makedirs(tmp_path / d / "infiles")
+        shutil.copy(TEMPLATE, tmp_path / d / "infiles" / "trimer_MVM")
+    _run(tmp_path / "one", flags + ["-T", "200", "-R", "false"])
+    _run(tmp_path / "two", flags + ["-T", "100", "-R", "false"])
+    _run(tmp_path / "two", flags + ["-T", "200", "-R", "true"])
+    a = open(tmp_path / "one" / "outfiles" / "restart_200.out").read()
+    b = open(tmp_path / "two" / "outfiles" / "restart_200.out").read()
+    assert a == b
+    ra = _rows(tmp_path / "one" / "outfiles" / "energy.out")[-1]
+    rb = _rows(tmp_path / "two" / "outfiles" / "energy.out")[-1]
+    assert ra[0] == 200 and ra == rb
+    # every double of the state is in the file: it reads back to the GPU state exactly
+    vals = np.loadtxt(tmp_path / "one" / "outfiles" / "restart_200.out", skiprows=1)
+    assert vals.shape == (20 * 105, 7)
+
+
+def _write_restart(path, pos, vel, a=0):
+    with open(path, "w") as f:
+        f.write(f"Velocities & Positions for {a}\n")
+        for i in range(len(pos)):
+            f.write(f"{i}  " + "  ".join(repr(float(x)) for x in (*vel[i], *pos[i])) + "\n")
+
+
+def _six_digits_close(printed, value):
+    """printed is value as the driver prints it (6 significant digits, default ostream)."""
+    if value == 0:
+        return printed == 0
+    e = np.floor(np.log10(abs(value)))
+    return abs(printed - value) <= 0.5 * 10 ** (e - 5) * (1 + 1e-9) + 1e-300
+
+
+def test_driver_energy_rows_match_oracle(tmp_path):
+    """energy.out rows of a run started from an injected state against the oracle's energies
+    after the same steps, at the printed 6 significant digits (src/md.cpp:339-346, 616-623)."""
+    from oracle_lib import Oracle
+    from souffle_states import make_state, read_template_xyz
+    os.makedirs(tmp_path / "infiles")
+    os.makedirs(tmp_path / "outfiles")
+    shutil.copy(TEMPLATE, tmp_path / "infiles" / "trimer_MVM")
+    o = Oracle(20, C_uM=800.0, c_mM=100.0, elj_att=2.1, dt=0.005, K=298.0, L=3.0, B=5)
+    pos, vel = make_state(read_template_xyz(TEMPLATE)[0], 20, o.box, 41, spacing=8.6)
+    _write_restart(tmp_path / "outfiles" / "restart_0.out", pos, vel)
+    _run(tmp_path, ["-T", "100", "-W", "50", "-w", "1000", "-M", "1000", "-R", "true"])
+    rows = _rows(tmp_path / "outfiles" / "energy.out")
+    assert [r[0] for r in rows] == [0, 0, 50, 100]
+    o.set_state(pos, vel)
+    N = 20 * 105
+    want = []
+    for n in (0, 1, 50, 50):
+        if n:
+            o.run(n)
+        ke, se, be, ne, ce, tp, tk = o.energies()
+        want.append([ke / N, se / N, be / N, ne / N, ce / N, (ke + se + be + ne + ce + tp + tk) / N,
+                     (be + se + ne + ce) / N, ke * 2 / (3 * N), tp / N, tk / N])
+    for r, w in zip(rows, want):
+        for k, (x, y) in enumerate(zip(r[1:11], w)):
+            assert _six_digits_close(x, y), (r[0], k, x, y)
+
+
+def test_driver_completion_fraction(tmp_path):
+    """cluster.dat in the reference's growth-kinetics layout (label = -b, time in ns, completion
+    fraction, spread), one row per movie frame, consistent with the frame's oligomer count."""
+    os.makedirs(tmp_path / "infiles")
+    shutil.copy(TEMPLATE, tmp_path / "infiles" / "trimer_MVM")
+    _run(tmp_path, ["-T", "200", "-W", "100", "-w", "1000", "-M", "50", "-R", "false", "--oligomers", "3.0",
+                    "--capsid", "20"])
+    out = tmp_path / "outfiles"
+    olig = _rows(out / "oligomers.out")
+    lines = open(out / "cluster.dat").read().splitlines()
+    assert lines[0] == "# label  time(ns)  cluster_avg  std"
+    rows = [l.split() for l in lines[1:]]
+    assert len(rows) == len(olig) == 5
+    si_time = 3.0958e-12  # data/model trimer_MVM at 298 K: sqrt(sigma^2 m / kT)
+    for r, og in zip(rows, olig):
+        assert r[0] == "20" and r[3] == "nan"
+        assert abs(float(r[1]) - og[0] * 0.005 * si_time / 1e-9) < 2e-6 + 1e-3 * float(r[1])
+        assert abs(float(r[2]) - (20.0 / og[1] - 1) / 19) < 1e-6
+
+
+def test_driver_stops_at_the_last_finite_state(tmp_path):
+    """Two one-bead subunits injected head-on (see test_gpu_resume.py) meet at one point in
+    library step 10: the driver reports it and writes the state after step 9 (iteration 8 of
+    its loop, iteration 0 being the first step) to lastfinite_8.out with its sidecar."""
+    from template_writer import write_template
+    os.makedirs(tmp_path / "infiles")
+    os.makedirs(tmp_path / "outfiles")
+    write_template(str(tmp_path / "infiles" / "point"), beads=[(0.0, 0.0, 0.0, 1, 0.0, 0.01, 1.0)])
+    _write_restart(tmp_path / "outfiles" / "restart_0.out", np.array([[-5.0, 0, 0], [5.0, 0, 0]]),
+                   np.array([[100.0, 0, 0], [-100.0, 0, 0]]))
+    r = subprocess.run([DRIVER, "-f", "point", "-S", "2", "-C", "50", "-c", "100", "-t", "0.005", "-q", "1",
+                        "-T", "50", "-W", "1000", "-w", "1000", "-M", "1000", "-R", "true"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert "non-finite at step 10" in r.stderr, r.stderr
+    vals = np.loadtxt(tmp_path / "outfiles" / "lastfinite_8.out", skiprows=1)
+    assert np.array_equal(vals[:, 4], [-0.5, 0.5]) and np.array_equal(vals[:, 1], [100.0, -100.0])
+    side = open(tmp_path / "outfiles" / "lastfinite_resume_8.txt").read().split()
+    assert side[:4] == ["iteration", "8", "step", "9"]

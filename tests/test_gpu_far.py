@@ -130,10 +130,13 @@ def test_model_gains_over_dropping_the_far_field(template, trained):
     assert err_none > 0
     # measured: the network's far field is ~6% off per bead, the dropped one 100%
     assert err_model < err_none / 8, (err_model, err_none)
-    # the audit measures the same quantity (integrator force after a step as denominator)
-    s.step(1)
+    # the audit measures the model's error relative to the far field it replaces: the forces
+    # differ only in the far field, so that is err_model / err_none at this state
     a = s.far_audit()
-    assert 0 < a < 1e-3
+    assert abs(a - err_model / err_none) <= 1e-6 * a, (a, err_model / err_none)
+    st = s.far_stats()
+    print("far-field relative error", a, "total-force relative error", st["last_err_force"], "dropped", err_none)
+    assert abs(st["last_err_force"] - err_model) <= 1e-3 * err_model
 
 
 def test_model_audit_falls_back(template, trained):
@@ -143,12 +146,32 @@ def test_model_audit_falls_back(template, trained):
     s.step(100)
     st = s.far_stats()
     assert st["audits"] == 1 and st["fell_back"] == 1 and st["using_model"] == 0
+    # the failed chunk was run again with the exact far field: the exact run's trajectory
+    ex, _, _, _ = _engine(template, 10.0)
+    ex.step(100)
+    assert all(np.array_equal(a, b) for a, b in zip(s.get_state(), ex.get_state()))
     s2, _, _, _ = _engine(template, 10.0)
     s2.set_far_model(m, tol=1.0)
     s2.step(200)
     st2 = s2.far_stats()
     assert st2["audits"] == 2 and st2["fell_back"] == 0 and st2["using_model"] == 1
-    assert st2["max_err"] < 1e-3
+    assert st2["max_err"] < 1.0 and st2["max_err_force"] < 1e-3
+
+
+def test_corrupted_model_trips_the_default_tolerance(template, trained):
+    """A model with perturbed weights fails the far-field audit at tol = 1e-3, although the same
+    difference is far below 1e-3 of the total force at c = 10 mM (where an audit against the
+    total force could never fail)."""
+    m, _ = trained
+    p0, mu, sd = m.params()
+    bad = sg.FarModel(template, RF, C_LO, C_HI, seed=2)
+    bad.set_params(p0 * np.float32(1.3), mu, sd)
+    s, _, _, _ = _engine(template, 10.0)
+    s.set_far_model(bad, tol=1e-3)
+    s.step(100)
+    st = s.far_stats()
+    assert st["fell_back"] == 1 and st["max_err"] > 1e-3, st
+    assert st["max_err_force"] < 1e-3, st
 
 
 def test_model_runs_deterministic(template, trained):
