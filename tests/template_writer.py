@@ -23,3 +23,31 @@ def write_template(path, beads, edges=(), faces=(), attractions=(), si_sigma=1e-
     with open(path, "w") as f:
         f.write("\n".join(L))
     return path
+
+
+def read_template(path):
+    """The sections of a template in the same format, as write_template takes them: (beads,
+    edges, faces, attractions, si_sigma, si_mass)."""
+    tok = open(path).read().split()
+    si_mass, si_sigma = float(tok[5]), float(tok[8])
+    i = tok.index("Particles") + 2
+    n = int(tok[i])
+    i += 9
+    beads = []
+    for k in range(n):
+        r = tok[i + 8 * k:i + 8 * k + 8]
+        beads.append((float(r[1]), float(r[2]), float(r[3]), int(r[4]), float(r[5]), float(r[6]), float(r[7])))
+    i = tok.index("Edges") + 2
+    n = int(tok[i])
+    i += 6
+    edges = [(int(tok[i + 5 * k + 1]), int(tok[i + 5 * k + 2]), int(tok[i + 5 * k + 3]), float(tok[i + 5 * k + 4]))
+             for k in range(n)]
+    i = tok.index("Triangles") + 2
+    n = int(tok[i])
+    i += 7
+    faces = [(int(tok[i + 6 * k + 1]), int(tok[i + 6 * k + 2]), int(tok[i + 6 * k + 3])) for k in range(n)]
+    i = tok.index("Attractions") + 2
+    n = int(tok[i])
+    i += 6
+    atts = [(int(tok[i + 5 * k + 1]), int(tok[i + 5 * k + 2])) for k in range(n)]
+    return beads, edges, faces, atts, si_sigma, si_mass

@@ -135,16 +135,17 @@ def test_deterministic_rerun():
 
 
 def test_fp32_pair_form_error():
-    t, p64, s64 = _system(20)
+    """FP32 pair arithmetic against the oracle's FP64 forces (measured ~1e-8 on this state)."""
     _, p32, s32 = _system(20, fp32=True)
-    pos, vel = _state(TEMPLATE, 20, p64.box, 9, 8.6)
-    s64.set_state(pos, vel)
+    pos, vel = _state(TEMPLATE, 20, p32.box, 9, 8.6)
+    o = _oracle(20)
+    o.set_state(pos, vel)
+    ref = o.forces()
     s32.set_state(pos, vel)
-    f64 = s64.forces()[0]
-    f32 = s32.forces()[0]
-    err = rel_norm(f32, f64)
-    assert err < 1e-3, err
-    assert err > 0
+    got = s32.forces()
+    assert 0 < rel_norm(got[0], ref[0]) < 1e-6, rel_norm(got[0], ref[0])
+    # bonded terms stay FP64
+    assert rel_norm(got[1], ref[1]) < 1e-10 and rel_norm(got[2], ref[2]) < 1e-10
 
 
 def test_replicas_match_single_runs():
