@@ -1240,12 +1240,17 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
 }
 
 // ------------------------------------------------------------------------------------------
-// Bending of one hinge: E = Kb (1 - |cos t|), cos t the cosine between the normals of the two
-// faces; equal to the reference's (grad S - (S/a0 grad a0 + S/a1 grad a1)) Kb / (4 a0 a1) with
-// S = 4 |A0.A1| (src/edge.cpp:69-188). With e = x1-x0, a = x2-x0, b = x3-x0, u = e x a,
-// v = e x b, c = u.v/(|u||v|): dc/du = g = (v/|v| - c u/|u|)/|u|, dc/dv = h likewise, and
-// dc/dx2 = g x e, dc/dx3 = h x e, dc/dx1 = a x g + b x h, dc/dx0 = -(sum).
-// Writes the 4 bead forces (12 doubles) and returns cos t.
+// Bending of one hinge, the reference's force (src/edge.cpp:69-188): Kb / (4 a0 a1) (grad P -
+// S (grad a0 / a0 + grad a1 / a1)), where S = 4 |A0.A1| (its face flip makes the product of the
+// two area vectors non-negative) and P is the unflipped product its gradient formulas
+// differentiate. With e = x1-x0, a = x2-x0, b = x3-x0, u = e x a, v = e x b (|u| = 2 a0,
+// |v| = 2 a1) and c = u.v/(|u||v|), P = -u.v, so the force is -Kb (grad c + (c + |c|) grad
+// ln(|u||v|)): for c <= 0 (faces folded less than 90 degrees) the gradient of the energy
+// Kb (1 - |c|); past 90 degrees it is not a gradient, and it is what the reference computes.
+// Per hinge it is the cross-product form below with dc/du replaced by
+// g = (v/|v| + |c| u/|u|)/|u| and dc/dv by h = (u/|u| + |c| v/|v|)/|v|: force on x2 = -Kb g x e,
+// x3: -Kb h x e, x1: -Kb (a x g + b x h), x0: minus their sum.
+// Writes the 4 bead forces (12 doubles) and returns c (the energy is Kb (1 - |c|)).
 __device__ __forceinline__ double hinge_forces(const double4* own, int4 hg, double box, double ibox, double Kb,
                                                double* out) {
   const double4 p0 = own[hg.x], p1 = own[hg.y], p2 = own[hg.z], p3 = own[hg.w];
@@ -1259,11 +1264,12 @@ __device__ __forceinline__ double hinge_forces(const double4* own, int4 hg, doub
   const double vx = ey * bz - ez * by, vy = ez * bx - ex * bz, vz = ex * by - ey * bx;
   const double iu = rsqrt(ux * ux + uy * uy + uz * uz), iv = rsqrt(vx * vx + vy * vy + vz * vz);
   const double c = (ux * vx + uy * vy + uz * vz) * iu * iv;
-  const double gx = (vx * iv - ux * (c * iu)) * iu, gy = (vy * iv - uy * (c * iu)) * iu,
-               gz = (vz * iv - uz * (c * iu)) * iu;
-  const double hx = (ux * iu - vx * (c * iv)) * iv, hy = (uy * iu - vy * (c * iv)) * iv,
-               hz = (uz * iu - vz * (c * iv)) * iv;
-  const double k = (c < 0) ? -Kb : Kb;
+  const double cn = -fabs(c);  // = c for the usual hinge (c <= 0)
+  const double gx = (vx * iv - ux * (cn * iu)) * iu, gy = (vy * iv - uy * (cn * iu)) * iu,
+               gz = (vz * iv - uz * (cn * iu)) * iu;
+  const double hx = (ux * iu - vx * (cn * iv)) * iv, hy = (uy * iu - vy * (cn * iv)) * iv,
+               hz = (uz * iu - vz * (cn * iv)) * iv;
+  const double k = -Kb;
   const double f2x = (gy * ez - gz * ey) * k, f2y = (gz * ex - gx * ez) * k, f2z = (gx * ey - gy * ex) * k;
   const double f3x = (hy * ez - hz * ey) * k, f3y = (hz * ex - hx * ez) * k, f3z = (hx * ey - hy * ex) * k;
   const double f1x = ((ay * gz - az * gy) + (by * hz - bz * hy)) * k;

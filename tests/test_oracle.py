@@ -8,14 +8,18 @@ fixes, each check chosen so that a plausible transcription mistake fails it:
     ordered lj_a rows, Yukawa size factor and cutoff;
   - energies against the closed forms with their cutoff shifts;
   - bending against E = Kb (1 - |n0 . n1|) computed from numpy face normals (value and
-    finite-difference gradient), for both face orientations;
+    finite-difference gradient), for both face orientations; past a 90 degree fold, where the
+    reference's force is no longer that gradient, against its closed form built from
+    finite-difference gradients of the face areas and the unflipped area product;
   - stretching against E = ks/2 (len0 - L)^2;
   - the whole trimer_MVM system: force = -grad(se + be + ne + ce) by finite differences;
   - Nose-Hoover: energy conservation with q = 1 and conservation of the extended energy with
     q = 5; BD: fluctuation-dissipation (free beads heat to kT = 1 at the drag rate);
   - the BD noise stream against an independent Python minstd_rand0 + generate_canonical;
   - the reference neighbour list contains every pair inside the force cutoffs;
-  - the FP64 mode agrees with the x87 long double mode to round-off.
+  - the FP64 mode agrees with the x87 long double mode to round-off;
+  - a run continued in a fresh oracle from (pos, vel, xi, eta, particle KE) follows the
+    uninterrupted run to round-off (orc_set_thermostat, used to continue GPU states).
 """
 import math
 import os
@@ -221,6 +225,54 @@ def test_bending_hinge_closed_form(tmp_path, faces, seed):
     assert np.allclose(F.sum(axis=0), 0, atol=1e-12)
 
 
+@pytest.mark.parametrize("faces", [[(0, 1, 2), (0, 1, 3)], [(0, 1, 2), (1, 0, 3)], [(2, 1, 0), (0, 3, 1)]])
+@pytest.mark.parametrize("angle", [60.0, 100.0, 150.0, 210.0, 300.0])
+def test_bending_folded_hinge(tmp_path, faces, angle):
+    """The reference's hinge force at any fold (src/edge.cpp:69-188): Kb/(4 a0 a1) (grad P -
+    |S| (grad a0/a0 + grad a1/a1)) with P = -(e x a).(e x b) the product its gradient formulas
+    differentiate (e = x1-x0, a = x2-x0, b = x3-x0; independent of the faces' winding), S = 4 A0.A1
+    and a0, a1 the face areas; gradients here by central differences of those scalars. Up to a 90
+    degree fold this is minus the gradient of Kb (1 - |n0.n1|); past it, it is not."""
+    base = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.5, 0.9, 0.0], [0.5, -0.9, 0.0]])
+    beads = [(*b, 1, 0.0, 1.0, 1.0) for b in base]
+    edges = [(0, 1, 1, 1.0), (0, 2, 0, 1.0), (1, 2, 0, 1.0), (0, 3, 0, 1.0), (1, 3, 0, 1.0)]
+    path = write_template(str(tmp_path / "hinge"), beads, edges, faces, [])
+    kb = 20.0
+    o = Oracle(1, template=path, C_uM=10.0, ks=0.0, kb=kb, L=7.5)
+    t = math.radians(angle)
+    x = base.copy()
+    x[3] = [0.5, -0.9 * math.cos(t), -0.9 * math.sin(t)]  # bead 3 turned about the hinge axis
+    x = x + np.random.default_rng(int(angle)).normal(scale=0.03, size=x.shape)
+    o.set_state(x)
+    F = o.forces()[2]
+
+    def uv(y):
+        e, a, b = y[1] - y[0], y[2] - y[0], y[3] - y[0]
+        return np.cross(e, a), np.cross(e, b)
+
+    def grad(f, y, h=1e-6):
+        g = np.zeros_like(y)
+        for i in range(4):
+            for c in range(3):
+                yp, ym = y.copy(), y.copy()
+                yp[i, c] += h
+                ym[i, c] -= h
+                g[i, c] = (f(yp) - f(ym)) / (2 * h)
+        return g
+
+    u, v = uv(x)
+    nu, nv = np.linalg.norm(u), np.linalg.norm(v)
+    gP = grad(lambda y: -np.dot(*uv(y)), x)
+    ga0 = grad(lambda y: np.linalg.norm(uv(y)[0]), x)
+    ga1 = grad(lambda y: np.linalg.norm(uv(y)[1]), x)
+    expect = kb / (nu * nv) * (gP - abs(np.dot(u, v)) * (ga0 / nu + ga1 / nv))
+    np.testing.assert_allclose(F, expect, rtol=0, atol=2e-7 * kb)
+    # the energy gradient agrees below a 90 degree fold only
+    c = -np.dot(u, v) / (nu * nv)
+    gE = grad(lambda y: kb * (1 - abs(np.dot(*uv(y))) / (np.linalg.norm(uv(y)[0]) * np.linalg.norm(uv(y)[1]))), x)
+    assert (np.abs(F + gE).max() < 1e-6 * kb) == (c > 0)
+
+
 def test_stretching_closed_form(tmp_path):
     beads = [(0, 0, 0, 1, 0.0, 1.0, 1.0), (1, 0, 0, 1, 0.0, 1.0, 1.0)]
     path = write_template(str(tmp_path / "bond"), beads, [(0, 1, 0, 1.0)], [], [])
@@ -310,6 +362,37 @@ def test_nose_hoover_extended_energy_conserved():
     assert max(abs(e - e0) for e in es) < 2e-3 * ke0
     # the thermostat is active: target T = 350/298 > initial T = 1 drives xi_0 negative
     assert min(xis) < -1e-3
+
+
+def test_set_thermostat_continues_a_run():
+    # a run stopped after 30 steps and continued in a fresh oracle from (pos, vel, xi, eta,
+    # particle KE) follows the uninterrupted run to round-off; a restart that resets the chain
+    # (the reference's restart) does not
+    xyz = read_template_xyz(TEMPLATE)[0]
+    kw = dict(q_chain=5, Q=10.0, K=350.0, C_uM=300.0, L=3.0, B=5)
+    a = Oracle(8, **kw)
+    pos, vel = make_state(xyz, 8, a.box, 5, spacing=9.0, perturb=0.02)
+    a.set_state(pos, vel)
+    a.run(30)
+    p30, v30, xi, eta = a.get_state()
+    assert np.all(np.abs(xi[:-1]) > 1e-6)  # the last link (Q = 0) stays at 0
+    ke = a.energies()[0]
+    a.run(30)
+    pa, va, xia, _ = a.get_state()
+    b = Oracle(8, **kw)
+    b.set_state(p30, v30)
+    b.set_thermostat(xi, eta, ke, step_index=30)
+    b.run(30)
+    pb, vb, xib, _ = b.get_state()
+    assert rel_norm(vb, va) < 1e-10 and rel_norm(pb, pa) < 1e-12
+    assert np.allclose(xib, xia, rtol=1e-9, atol=1e-14)
+    c = Oracle(8, **kw)
+    c.set_state(p30, v30)
+    c.step_index = 30
+    c.run(30)
+    assert rel_norm(c.get_state()[1], va) > 1e-6
+    with pytest.raises(RuntimeError):
+        b.set_thermostat(xi[:3], eta[:3], ke)
 
 
 def test_bd_fluctuation_dissipation(tmp_path):
