@@ -254,6 +254,38 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+
+
+def flops_per_launch(kernel, workload):
+    """(fp64, fp32, bf16_mfma) FLOPs per launch of `kernel` on `workload` from the committed
+    rocprofv3 PMC summary, or None."""
+    path = os.path.join(REPO, "profiles", "flops_per_launch.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and f[0] == kernel and f[1] == workload:
+            return tuple(float(x) for x in f[2:])
+    return None
+
+
+def flop_rate(kernel, workload, ms):
+    """Counted FLOP rates of one kernel against the MI355X peaks, from its FLOPs per launch
+    (PMC) and its event-timed mean launch duration."""
+    f = flops_per_launch(kernel, workload)
+    if f is None or not ms or ms <= 0:
+        return None
+    out = {"kernel_ms": ms, "workload": workload}
+    for name, flops, peak in (("fp64", f[0], FP64_PEAK_TFLOPS), ("fp32", f[1], FP32_PEAK_TFLOPS),
+                              ("bf16_mfma", f[2], BF16_PEAK_TFLOPS)):
+        if flops > 0:
+            tf = flops / (ms * 1e-3) / 1e12
+            out[name] = {"flop_per_launch": flops, "tflops": tf, "peak_tflops": peak, "frac": tf / peak}
+    return out
+
+
 def roofline_entry(nb, n_list, k_rebuild, t_force, value, traffic):
     b_force = force_kernel_bytes_per_bead(n_list)
     achieved = b_force * nb / (t_force * 1e-3) / 1e9 if t_force > 0 else 0.0
@@ -366,8 +398,8 @@ def run_surrogate(args, steps, warmup):
     arithmetic with the far field from the network (trained here on the GPU), from the same
     seeded state. The accuracy bar is 1e-3 relative RMS error of the total force; the library's
     audit measures the network's error relative to the far field it replaces, so its tolerance is
-    set to 1e-3 / (the far field's share of the force on the start state), the far-field error
-    that keeps the total-force error at 1e-3."""
+    set to 1e-3 / (the far field's share of the force on the plain run's warmed state), the
+    far-field error that keeps the total-force error at 1e-3."""
     import numpy as np
     import souffle_gpu as sg
     t = sg.Template(TEMPLATE)
@@ -388,8 +420,15 @@ def run_surrogate(args, steps, warmup):
         pos, vel = t.make_state(ns, p.box, p.T, 2000)
         s = sg.System(t, ns, p)
         s.set_state(pos, vel)
+        if args.no_graph:
+            s.set_timing(True)  # direct launches throughout (for kernel traces and counters)
+        if use_model:
+            s.set_far_model(model, tol=tol)
+        s.step(warmup)
         if not use_model:
-            # the far field's share of the force and the network's error on the start state
+            # the far field's share of the force and the network's error on the warmed state (on
+            # the start state the subunits sit at their template geometry, the bonded forces are
+            # near zero and the far field's share is tens of times its share in a running system)
             f64 = s.forces()[0]
             s.far_drop(True)
             fn = s.forces()[0]
@@ -398,19 +437,17 @@ def run_surrogate(args, steps, warmup):
             fm = s.forces()[0]
             s.set_far_model(None)
             share = rel(fn, f64)
-            out["start_state"] = {"far_field_share_of_force": share, "model_err_rel_far_field": rel(fm, f64) / share,
-                                  "model_err_rel_force": rel(fm, f64)}
+            out["warmed_state"] = {"steps": warmup, "far_field_share_of_force": share,
+                                   "model_err_rel_far_field": rel(fm, f64) / share, "model_err_rel_force": rel(fm, f64)}
             tol = 1e-3 / share
             out["audit_tol_rel_far_field"] = tol
-        else:
-            s.set_far_model(model, tol=tol)
-        s.step(warmup)
         t1 = time.perf_counter()
         s.step(steps)
         wall = time.perf_counter() - t1
         s.set_timing(True)
         s.step(min(steps, 500))
-        s.set_timing(False)
+        if not args.no_graph:
+            s.set_timing(False)
         st = s.stats()
         out[name] = {"bead_steps_per_s": s.n_beads * steps / wall, "ms_per_step": 1e3 * wall / steps,
                      "kernels_ms": {"k_integrate": st["k_integrate_ms"], "k_rebuild": st["k_rebuild_ms"],
@@ -458,6 +495,11 @@ def bench_surrogate(args):
         "plain_fp64": res["fp64_exact"], "surrogate_fp32": res["fp32_model"],
         "speedup_vs_fp64_exact": res["speedup_vs_fp64_exact"], "force_rel_rms_err": res["force_rel_rms_err"],
         "far_field_share_of_force": res["far_field_share_of_force"], "training": res["train"],
+        "audit_tol_rel_far_field": res["audit_tol_rel_far_field"], "warmed_state": res["warmed_state"],
+        "flop_rates": {k: v for k, v in (
+            ("k_far_exact", flop_rate("k_far_exact", "trimer5000_uncut", res["fp64_exact"]["kernels_ms"]["k_far"])),
+            ("k_far_mlp", flop_rate("k_far_mlp", "trimer5000_uncut", res["fp32_model"]["kernels_ms"]["k_far"])))
+            if v is not None and args.sur_subunits == 5000},
         "training_wall_s": res["train_wall_s"],
     }
 
@@ -544,6 +586,11 @@ def bench_md(args):
         # the event-timed pass launches the step kernels directly; each mean also holds the
         # cost of its two event records (~3 us against a kernel trace of the same launches)
         "kernel_ms_note": "HIP-event means over direct launches, event cost included",
+        # counted FLOP rates (rocprofv3 PMC FLOPs per launch / event-timed mean; issue-slot FLOPs)
+        "flop_rates": {k: v for k, v in (
+            ("k_force", flop_rate("k_force", "trimer2000_nh_fp64", t_force) if args.subunits == 2000 else None),
+            ("k_integrate", flop_rate("k_integrate", "trimer2000_nh_fp64", st["k_integrate_ms"])
+             if args.subunits == 2000 else None)) if v is not None},
         # the untimed pass before the timed window (graph capture of its chunk lengths, the tuner's
         # list build)
         "first_window": md["first_window"],
@@ -569,8 +616,12 @@ def bench_md(args):
             "force_rel_rms_err": sur["force_rel_rms_err"],
             "far_field_share_of_force": sur["far_field_share_of_force"],
             "model_err_rel_far_field": sur["model_err_rel_far_field"],
-            "audit_tol_rel_far_field": sur["audit_tol_rel_far_field"], "start_state": sur["start_state"],
+            "audit_tol_rel_far_field": sur["audit_tol_rel_far_field"], "warmed_state": sur["warmed_state"],
             "k_far_ms": {"exact": sur["fp64_exact"]["kernels_ms"]["k_far"], "model": sur["fp32_model"]["kernels_ms"]["k_far"]},
+            "flop_rates": {k: v for k, v in (
+                ("k_far_exact", flop_rate("k_far_exact", "trimer5000_uncut", sur["fp64_exact"]["kernels_ms"]["k_far"])),
+                ("k_far_mlp", flop_rate("k_far_mlp", "trimer5000_uncut", sur["fp32_model"]["kernels_ms"]["k_far"])))
+                if v is not None and args.sur_subunits == 5000},
             "audits": sur["fp32_model"]["audits"], "training_wall_s": sur["train_wall_s"],
             "val_rel_err_force": sur["train"]["val_rel_err_force"]}
     for k in ("system",):

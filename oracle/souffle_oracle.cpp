@@ -843,6 +843,19 @@ int orc_set_state(void* hv, const double* pos, const double* vel) {
   });
 }
 
+// Sets the chain (xi, eta for each of the n thermostats) and the particle kinetic energy that
+// the next step's first chain update uses (src/md.cpp:400-403 reads the value measured at the
+// end of the previous step), after orc_set_state: together they continue a run that another
+// program stopped mid-trajectory. n must equal the chain length.
+int orc_set_thermostat(void* hv, const double* xi, const double* eta, int n, double particle_ke) {
+  return dispatch((OrcHandle*)hv, [&](auto& s) {
+    if (n != (int)s.bath.size()) return -1;
+    for (int k = 0; k < n; k++) { s.bath[k].xi = xi[k]; s.bath[k].eta = eta[k]; }
+    s.particle_ke = particle_ke;
+    return 0;
+  });
+}
+
 int orc_get_state(void* hv, double* pos, double* vel, double* xi, double* eta) {
   return dispatch((OrcHandle*)hv, [&](auto& s) {
     for (int i = 0; i < s.N; i++) {

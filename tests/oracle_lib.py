@@ -26,6 +26,7 @@ def load():
         lib.orc_params.argtypes = [ctypes.c_void_p, _D]
         lib.orc_set_state.argtypes = [ctypes.c_void_p, _D, _D]
         lib.orc_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
+        lib.orc_set_thermostat.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, ctypes.c_double]
         lib.orc_forces.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D]
         lib.orc_energies.argtypes = [ctypes.c_void_p, _D, _D]
         lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int]
@@ -83,6 +84,15 @@ class Oracle:
         self.step_index = 0
         if rc:
             raise RuntimeError("oracle neighbour list overflow")
+
+    def set_thermostat(self, xi, eta, particle_ke, step_index=0):
+        """Continue another program's NH run: chain xi/eta, the kinetic energy the next step's
+        first chain update reads, and the loop index of the next step (list-rebuild schedule)."""
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        eta = np.ascontiguousarray(eta, dtype=np.float64)
+        if load().orc_set_thermostat(self.ptr, _dp(xi), _dp(eta), len(xi), float(particle_ke)):
+            raise RuntimeError("chain length differs from the oracle's")
+        self.step_index = step_index
 
     def get_state(self):
         pos = np.zeros((self.N, 3))

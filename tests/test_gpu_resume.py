@@ -63,6 +63,28 @@ def test_resume_is_bit_continuous(brownian):
     assert not np.array_equal(d.get_state()[1], va)
 
 
+def test_resume_record_continues_in_the_oracle():
+    """The resume record means what the reference's thermostat state means: the oracle, given
+    positions, velocities and the record's xi, eta and KE, continues the GPU run at 1e-8."""
+    t = sg.Template(TEMPLATE)
+    a, p = _system(t, 20)
+    pos, vel = make_state(read_template_xyz(TEMPLATE)[0], 20, p.box, 34, spacing=8.6)
+    a.set_state(pos, vel)
+    a.step(100)
+    mid_pos, mid_vel = a.get_state()
+    rec = a.get_resume(with_force=False)[0][0]
+    a.step(100)
+    ga, gv = a.get_state()
+    gxi, _ = a.thermostat()
+    o = Oracle(20, B=1, **CFG0)
+    o.set_state(mid_pos, mid_vel)
+    o.set_thermostat(rec["xi"], rec["eta"], rec["ke"], step_index=100)
+    o.run(100)
+    op, ov, oxi, _ = o.get_state()
+    assert rel_norm(gv, ov) < 1e-8 and rel_norm(ga, op) < 1e-8
+    assert np.allclose(gxi[:4], oxi[:4], rtol=1e-8, atol=1e-12)
+
+
 def test_resume_checks_its_records():
     t = sg.Template(TEMPLATE)
     a, p = _system(t, 20, True)
