@@ -103,6 +103,10 @@ struct sf_handle {
       if (any_att) rlj = std::max(rlj, p.ecut);
       rlj += p.skin;
       P.rl_lj = rlj;
+      // list entries were within their list radius at the build and each bead has moved less
+      // than skin/2 since: no entry is farther than the larger list radius plus the skin (the
+      // uncut Yukawa's near lists reach far_rf: no bound is used there)
+      P.pair_reach = P.split ? 1e300 : std::max(rlj, p.ecut_el + p.skin) + p.skin + 1e-2;
       P.sub_rc_lj2 = (rlj + 2 * rb) * (rlj + 2 * rb);
       const double rql = p.ecut_el + p.skin;
       P.sub_rc_q2 = (rql + 2 * rq) * (rql + 2 * rq);
@@ -685,6 +689,22 @@ std::vector<double4> tagged_positions(const sf_handle* h, const double* pos) {
   return hp;
 }
 
+// Records per replica whether every given coordinate lies inside [-box/2, box/2] (the steps keep
+// it so: one periodic shift per step), which lets the force kernel skip the image shift of list
+// neighbours of beads away from the box faces; uploads the replica constants.
+int note_wrapped(sf_handle* h, const double* pos) {
+  const int np = h->t.np;
+  for (int r = 0; r < h->R; r++) {
+    RepParams& P = h->rp[r];
+    bool in = true;
+    for (size_t i = (size_t)P.s0 * np; in && i < (size_t)(P.s0 + P.ns) * np; i++)
+      for (int c = 0; c < 3; c++)
+        if (!(std::fabs(pos[3 * i + c]) <= P.hbox)) { in = false; break; }
+    P.wrapped = in ? 1 : 0;
+  }
+  return h->upload_rp();
+}
+
 // Uploads a state and starts every replica: from scratch (res == nullptr: thermostat zero, BD
 // engine seeded, step 0) or from a resume record per replica.
 int set_state_impl(sf_handle* h, const double* pos, const double* vel, const sf_resume* res, const double* tforce) {
@@ -719,6 +739,7 @@ int set_state_impl(sf_handle* h, const double* pos, const double* vel, const sf_
     rs[r].resume = 1;
   }
   HIPCHK(hipMemcpyAsync(h->d.pos, hp.data(), nb * sizeof(double4), hipMemcpyHostToDevice, h->st));
+  if (int rc = note_wrapped(h, pos)) return rc;
   HIPCHK(hipMemcpyAsync(h->d.vx, hv.data(), 3 * nb * sizeof(double), hipMemcpyHostToDevice, h->st));
   HIPCHK(hipMemsetAsync(h->d.fx, 0, 3 * nb * sizeof(double), h->st));
   HIPCHK(hipMemsetAsync(h->d.gx, 0, 3 * nb * sizeof(double), h->st));
@@ -780,6 +801,7 @@ int sf_set_positions(sf_handle* h, const double* pos) {
   if (!h || !pos) { set_error("null argument"); return SF_E_ARG; }
   std::vector<double4> hp = tagged_positions(h, pos);
   HIPCHK(hipMemcpyAsync(h->d.pos, hp.data(), h->Nb * sizeof(double4), hipMemcpyHostToDevice, h->st));
+  if (int rc = note_wrapped(h, pos)) return rc;
   if (int rc = launch_moved(h->d, h->st)) return rc;
   HIPCHK(hipStreamSynchronize(h->st));
   return SF_OK;

@@ -34,6 +34,18 @@ __device__ __forceinline__ double min_img(double v, double box, double hbox) {
 __device__ __forceinline__ double min_img_near(double v, double box, double ibox) {
   return fma(-rint(v * ibox), box, v);
 }
+// min_img_near where the caller may know that no shift is needed: a difference of two raw
+// coordinates less than half a box apart (two beads of a subunit that does not straddle the
+// periodic boundary; a bead away from the box faces and a list neighbour), for which the shift
+// rounds to zero and min_img_near returns the difference itself, bit for bit.
+template <bool WRAP>
+__device__ __forceinline__ double near_diff(double v, double box, double ibox) {
+  return WRAP ? min_img_near(v, box, ibox) : v;
+}
+template <bool B>
+struct Flag {
+  static constexpr bool value = B;
+};
 // Branch-free single shift in FP32 for the list builds: the same value as the conditional shift
 // for a separation far below half a box; elsewhere both give a separation of about half a box,
 // so a test against a list radius (< half a box) decides the same.
@@ -1251,15 +1263,16 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
 // g = (v/|v| + |c| u/|u|)/|u| and dc/dv by h = (u/|u| + |c| v/|v|)/|v|: force on x2 = -Kb g x e,
 // x3: -Kb h x e, x1: -Kb (a x g + b x h), x0: minus their sum.
 // Writes the 4 bead forces (12 doubles) and returns c (the energy is Kb (1 - |c|)).
+template <bool WRAP>
 __device__ __forceinline__ double hinge_forces(const double4* own, int4 hg, double box, double ibox, double Kb,
                                                double* out) {
   const double4 p0 = own[hg.x], p1 = own[hg.y], p2 = own[hg.z], p3 = own[hg.w];
-  const double ex = min_img_near(p1.x - p0.x, box, ibox), ey = min_img_near(p1.y - p0.y, box, ibox),
-               ez = min_img_near(p1.z - p0.z, box, ibox);
-  const double ax = min_img_near(p2.x - p0.x, box, ibox), ay = min_img_near(p2.y - p0.y, box, ibox),
-               az = min_img_near(p2.z - p0.z, box, ibox);
-  const double bx = min_img_near(p3.x - p0.x, box, ibox), by = min_img_near(p3.y - p0.y, box, ibox),
-               bz = min_img_near(p3.z - p0.z, box, ibox);
+  const double ex = near_diff<WRAP>(p1.x - p0.x, box, ibox), ey = near_diff<WRAP>(p1.y - p0.y, box, ibox),
+               ez = near_diff<WRAP>(p1.z - p0.z, box, ibox);
+  const double ax = near_diff<WRAP>(p2.x - p0.x, box, ibox), ay = near_diff<WRAP>(p2.y - p0.y, box, ibox),
+               az = near_diff<WRAP>(p2.z - p0.z, box, ibox);
+  const double bx = near_diff<WRAP>(p3.x - p0.x, box, ibox), by = near_diff<WRAP>(p3.y - p0.y, box, ibox),
+               bz = near_diff<WRAP>(p3.z - p0.z, box, ibox);
   const double ux = ey * az - ez * ay, uy = ez * ax - ex * az, uz = ex * ay - ey * ax;
   const double vx = ey * bz - ez * by, vy = ez * bx - ex * bz, vz = ex * by - ey * bx;
   const double iu = rsqrt(ux * ux + uy * uy + uz * uz), iv = rsqrt(vx * vx + vy * vy + vz * vz);
@@ -1346,14 +1359,14 @@ struct PairCtx {
 // Pair term of entry c0 + le of the subunit's concatenated lists (LJ entries, then Yukawa
 // entries, bead by bead) on its bead from the neighbour at pj (src/forces.cpp:67-139); the
 // force components go to pf[le], pf[kChunk + le], pf[2 kChunk + le].
-template <typename PT, int kChunk>
+template <typename PT, int kChunk, bool WRAP>
 __device__ __forceinline__ void pair_entry(int le, int c0, double4 pj, PairCtx<PT> c, PT* pf) {
   const int i = c.owner[le];
   const int k = c0 + le - c.off[i];
   const int nl = c.nlj[i];
   const double4 p0 = c.own[i];
-  const PT dx = (PT)min_img_near(p0.x - pj.x, c.box, c.ibox), dy = (PT)min_img_near(p0.y - pj.y, c.box, c.ibox),
-           dz = (PT)min_img_near(p0.z - pj.z, c.box, c.ibox);
+  const PT dx = (PT)near_diff<WRAP>(p0.x - pj.x, c.box, c.ibox), dy = (PT)near_diff<WRAP>(p0.y - pj.y, c.box, c.ibox),
+           dz = (PT)near_diff<WRAP>(p0.z - pj.z, c.box, c.ibox);
   const PT r2 = dx * dx + dy * dy + dz * dz;
   const int lj = tag_loc(pj.w);
   PT f;
@@ -1431,6 +1444,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   __shared__ int off_s[kBlock + 1];
   __shared__ int nlj_s[kBlock];
   __shared__ unsigned char owner_s[kChunk];
+  __shared__ int wflag[kBlock / 64];
   const int np = d.topo.np, nh = d.topo.nh, nq = d.topo.nq;
   double4* own = reinterpret_cast<double4*>(smem);
   double* hc = reinterpret_cast<double*>(own + np);          // [nh][9] forces on hinge beads 1-3
@@ -1451,8 +1465,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   // one round of independent loads: own position, velocity, list sizes, the packed edge and
   // hinge rows, the template bead tables and the Yukawa tables
   int qi = -1, nlj = 0, nqe = 0;
+  double4 p_own = cs;
   if (active) {
-    own[tid] = d.pos[gi];
+    p_own = d.pos[gi];
+    own[tid] = p_own;
     tb.sigma[tid] = d.topo.sigma[tid];
     tb.type[tid] = d.topo.type[tid];
     qi = d.topo.qslot[tid];
@@ -1486,7 +1502,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     if ((tid & 63) >= o) incl += v;
   }
   if ((tid & 63) == 63) off_s[kBlock - 1 - (tid >> 6)] = incl;  // wave totals, parked at the end
+  // differences that need no image shift (near_diff): the subunit's beads all within a quarter
+  // box of its centre bead on every axis (bonded vectors), and, with positions kept inside the
+  // box, all beads at least the list reach from the box faces (list neighbours, whose images are
+  // then their raw positions)
+  {
+    const double q = 0.5 * hbox, lim = hbox - P.pair_reach;
+    const bool ok_b = fabs(p_own.x - cs.x) < q && fabs(p_own.y - cs.y) < q && fabs(p_own.z - cs.z) < q;
+    const bool ok_p = fabs(p_own.x) <= lim && fabs(p_own.y) <= lim && fabs(p_own.z) <= lim;
+    const unsigned long long vb = __ballot(ok_b), vp = __ballot(ok_p);
+    if ((tid & 63) == 0) wflag[tid >> 6] = (vb == ~0ull ? 1 : 0) | (vp == ~0ull ? 2 : 0);
+  }
   __syncthreads();
+  const int wf = __builtin_amdgcn_readfirstlane(wflag[0] & wflag[1]);
+  const bool bond_wrap = !(wf & 1), pair_wrap = !(wf & 2) || !P.wrapped;
   const int wave0_total = off_s[kBlock - 1];
   const int off_i = incl - n_i + ((tid >> 6) ? wave0_total : 0);
   const int total = wave0_total + off_s[kBlock - 2];
@@ -1518,33 +1547,40 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     }
   }
   // bending hinges -> LDS (the rows of a thread's first two hinges were loaded with the tables)
-  for (int h = tid; h < nh; h += kBlock) {
-    const int4 hg = (h == tid) ? hg0 : (h == tid + kBlock) ? hg1 : reinterpret_cast<const int4*>(d.topo.hinge)[h];
-    double hf[12];
-    hinge_forces(own, hg, box, ibox, kb, hf);
+  auto hinge_loop = [&](auto wrap) {
+    for (int h = tid; h < nh; h += kBlock) {
+      const int4 hg = (h == tid) ? hg0 : (h == tid + kBlock) ? hg1 : reinterpret_cast<const int4*>(d.topo.hinge)[h];
+      double hf[12];
+      hinge_forces<decltype(wrap)::value>(own, hg, box, ibox, kb, hf);
 #pragma unroll
-    for (int k = 0; k < 9; k++) hc[9 * h + k] = hf[3 + k];
-  }
+      for (int k = 0; k < 9; k++) hc[9 * h + k] = hf[3 + k];
+    }
+  };
+  if (bond_wrap) hinge_loop(Flag<true>{}); else hinge_loop(Flag<false>{});
   double4 pre0 = make_double4(0, 0, 0, 0), pre1 = pre0;  // no indexed array
   if (pj0 >= 0) pre0 = d.pos[pj0];
   if (kPre > 1 && pj1 >= 0) pre1 = d.pos[pj1];
   // stretching (src/bead.cpp:17-23) over the bead's edge row
   double sx = 0, sy = 0, sz = 0;
   double4 pi = make_double4(0, 0, 0, 0);
-  if (active) {
-    pi = own[tid];
+  auto stretch_loop = [&](auto wrap) {
+    constexpr bool W = decltype(wrap)::value;
 #pragma unroll
     for (int k = 0; k < kMaxDeg; k++) {
       if (k >= ecnt) break;
       const int o = ((k < 4 ? erow.x : erow.y) >> (8 * (k & 3))) & 0xff;
       const double l0 = (k & 1) ? len[k >> 1].y : len[k >> 1].x;
       const double4 po = own[o];
-      const double dx = min_img_near(pi.x - po.x, box, ibox), dy = min_img_near(pi.y - po.y, box, ibox),
-                   dz = min_img_near(pi.z - po.z, box, ibox);
+      const double dx = near_diff<W>(pi.x - po.x, box, ibox), dy = near_diff<W>(pi.y - po.y, box, ibox),
+                   dz = near_diff<W>(pi.z - po.z, box, ibox);
       const double L2 = dx * dx + dy * dy + dz * dz, iL = rsqrt(L2), L = L2 * iL;
       const double c = (l0 - L) * ks * iL;
       sx += dx * c; sy += dy * c; sz += dz * c;
     }
+  };
+  if (active) {
+    pi = own[tid];
+    if (bond_wrap) stretch_loop(Flag<true>{}); else stretch_loop(Flag<false>{});
   }
   // bending gather (fixed hinge order); hinge bead 0 takes minus the sum of the other three,
   // formed as the hinge computes it
@@ -1574,6 +1610,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     ctx.centres = d.centres; ctx.cs = cs; ctx.box = box; ctx.hbox = hbox; ctx.ibox = ibox; ctx.far_rf2 = P.far_rf2;
     ctx.ecut2 = (PT)P.ecut2; ctx.elj = (PT)P.elj_att; ctx.kappa = (PT)P.kappa; ctx.ecut_el2 = (PT)P.ecut_el2;
     ctx.nq = nq; ctx.split = P.split;
+    auto entry = [&](int le, int c0, double4 pj) {
+      if (pair_wrap) pair_entry<PT, kChunk, true>(le, c0, pj, ctx, pf);
+      else pair_entry<PT, kChunk, false>(le, c0, pj, ctx, pf);
+    };
     for (int c0 = 0; c0 < total; c0 += kChunk) {
       __syncthreads();
       const int c1 = min(total, c0 + kChunk);
@@ -1582,8 +1622,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
       __syncthreads();
       int e_next = c0 + tid;
       if (c0 == 0) {  // the prefetched positions, one call per entry slot (no indexed array)
-        if (tid < c1) pair_entry<PT, kChunk>(tid, 0, pre0, ctx, pf);
-        if (kPre > 1 && tid + kBlock < c1) pair_entry<PT, kChunk>(tid + kBlock, 0, pre1, ctx, pf);
+        if (tid < c1) entry(tid, 0, pre0);
+        if (kPre > 1 && tid + kBlock < c1) entry(tid + kBlock, 0, pre1);
         e_next = tid + kPre * kBlock;
       }
       {
@@ -1593,7 +1633,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
           const int nl = nlj_s[i];
           const int j = (k < nl) ? d.lj_list[(size_t)k * d.Nb + s * np + i]
                                  : d.q_list[(size_t)(k - nl) * d.Nq + (s * nq + tb.qslot[i])];
-          pair_entry<PT, kChunk>(e - c0, c0, d.pos[j], ctx, pf);
+          entry(e - c0, c0, d.pos[j]);
         }
       }
       __syncthreads();
@@ -1928,7 +1968,7 @@ __global__ __launch_bounds__(kBlock) void k_energy(Dev d) {
   for (int h = tid; h < nh; h += kBlock) {
     double tmp[12];
     const int4 hg = reinterpret_cast<const int4*>(d.topo.hinge)[h];
-    const double c = hinge_forces(own, hg, box, P.ibox, P.kb, tmp);
+    const double c = hinge_forces<true>(own, hg, box, P.ibox, P.kb, tmp);
     he[h] = P.kb * 0.25 * (1 - fabs(c));  // src/edge.cpp:64-66 after the flip: n0.n1 = |cos|
   }
   __syncthreads();

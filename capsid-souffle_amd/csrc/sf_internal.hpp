@@ -117,6 +117,8 @@ struct RepParams {
   double r_bound;                // its square root
   double r_bound_q2;             // the same bound^2 for charged beads
   double rl_lj;                  // largest LJ/WCA list radius (cutoff + skin) over bead pairs
+  double pair_reach;             // bound on the distance of any list entry (list radius + skin)
+  int wrapped;                   // all positions lie in [-box/2, box/2] (kept so by the steps)
   int grid_ok;                   // 1 if the local build grid fits in half a box
   double inv_nb;                 // 1.0 / beads of the replica
   double Qb[kMaxChain];          // bath masses
