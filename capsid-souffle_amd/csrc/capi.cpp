@@ -33,6 +33,17 @@ constexpr double kNoCut2 = 1e30;     // squared "cutoff" of the uncut Yukawa's b
 
 }  // namespace
 
+// Square of the displacement that triggers a list build. Positions at the last build are kept in
+// FP32, so a measured displacement can fall short of the true one by the rounding of three
+// coordinates (2^-24 relative each; coordinates stay within a box of the origin); the threshold
+// is lowered by twice that bound (a build comes at most that much earlier), so no bead moves
+// more than skin/2 between builds and every pair within its cutoff stays in the lists.
+static double half_skin_test2(double skin, double box) {
+  const double margin = 2.0 * std::sqrt(3.0) * box * std::ldexp(1.0, -24);
+  const double h = 0.5 * skin - margin;
+  return h > 0 ? h * h : 0.0;
+}
+
 struct sf_handle {
   sf_template t;
   int R = 0, Ns = 0, Nb = 0, Nq = 0;
@@ -133,7 +144,7 @@ struct sf_handle {
     prm[r].skin = skin;
     RepParams& P = rp[r];
     P.skin = skin;
-    P.half_skin2 = 0.25 * skin * skin;
+    P.half_skin2 = half_skin_test2(skin, P.box);
     P.rc_q_list2 = P.split ? kNoCut2 : (prm[r].ecut_el + skin) * (prm[r].ecut_el + skin);
   }
 
@@ -605,7 +616,7 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
     P.ecut_el = p.ecut_el; P.ecut_el2 = p.ecut_el * p.ecut_el;
     P.kappa = p.kappa; P.lb = p.lb; P.elj_att = p.elj_att;
     P.ks = p.ks; P.kb = p.kb; P.dt = p.dt; P.damp = p.damp; P.T = p.T;
-    P.skin = p.skin; P.half_skin2 = 0.25 * p.skin * p.skin;
+    P.skin = p.skin; P.half_skin2 = half_skin_test2(p.skin, P.box);
     P.rc_q_list2 = (p.ecut_el + p.skin) * (p.ecut_el + p.skin);
     P.split = split;
     P.far_rf2 = h->far_rf[r] * h->far_rf[r];

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
   const double box = P.box, hbox = P.hbox, dt = P.dt, half_skin2 = P.half_skin2;
   const double m = d.topo.mass[l];
   const double mvx = S.mean_v[0], mvy = S.mean_v[1], mvz = S.mean_v[2];
-  const double4 q = d.xb[b];
+  const float4 q = d.xb[b];
   double vx = d.vx[b] - mvx, vy = d.vy[b] - mvy, vz = d.vz[b] - mvz;
   const double fx = d.fx[b], fy = d.fy[b], fz = d.fz[b];
   if (MODE == MODE_MD) {
@@ -202,8 +202,8 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
   d.pos[b] = p;
   d.vx[b] = vx; d.vy[b] = vy; d.vz[b] = vz;
   const double ibox = P.ibox;
-  const double ddx = min_img_near(p.x - q.x, box, ibox), ddy = min_img_near(p.y - q.y, box, ibox),
-               ddz = min_img_near(p.z - q.z, box, ibox);
+  const double ddx = min_img_near(p.x - (double)q.x, box, ibox), ddy = min_img_near(p.y - (double)q.y, box, ibox),
+               ddz = min_img_near(p.z - (double)q.z, box, ibox);
   if (ddx * ddx + ddy * ddy + ddz * ddz > half_skin2) S.rebuild = 1;  // same value from any thread
   if (l == d.topo.centre) d.centres[s] = p;
   const int qs = d.topo.qslot[l];
@@ -225,9 +225,10 @@ __global__ __launch_bounds__(256) void k_moved(Dev d) {
   const int s = b / np, l = b - s * np;
   const int r = rep_of(d, s);
   const RepParams& P = d.rp[r];
-  const double4 p = d.pos[b], q = d.xb[b];
-  const double ddx = min_img(p.x - q.x, P.box, P.hbox), ddy = min_img(p.y - q.y, P.box, P.hbox),
-               ddz = min_img(p.z - q.z, P.box, P.hbox);
+  const double4 p = d.pos[b];
+  const float4 q = d.xb[b];
+  const double ddx = min_img(p.x - (double)q.x, P.box, P.hbox), ddy = min_img(p.y - (double)q.y, P.box, P.hbox),
+               ddz = min_img(p.z - (double)q.z, P.box, P.hbox);
   if (!(ddx * ddx + ddy * ddy + ddz * ddz <= P.half_skin2)) d.rs[r].rebuild = 1;  // same value from any thread
   if (l == d.topo.centre) d.centres[s] = p;
   const int qs = d.topo.qslot[l];
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
     const int qi = d.topo.qslot[tid];
     d.lj_cnt[gi] = cnt_lj;
     if (qi >= 0) d.q_cnt[s * nq + qi] = cnt_q;
-    d.xb[gi] = pi;
+    d.xb[gi] = make_float4((float)pi.x, (float)pi.y, (float)pi.z, 0.0f);
     if (cnt_lj > d.lj_cap) atomicMax(&S.need_lj, cnt_lj);
     if (cnt_q > d.q_cap) atomicMax(&S.need_q, cnt_q);
     const double dx = min_img(pi.x - cs.x, box, hbox), dy = min_img(pi.y - cs.y, box, hbox),
@@ -1238,7 +1239,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   }
   if (active) {
     d.lj_cnt[gi] = cnt_lj;
-    d.xb[gi] = pi;
+    d.xb[gi] = make_float4((float)pi.x, (float)pi.y, (float)pi.z, 0.0f);
     if (cnt_lj > d.lj_cap) atomicMax(&S.need_lj, cnt_lj);
   }
   // list sizes of this subunit for the statistics

@@ -110,7 +110,7 @@ struct RepParams {
   double box, hbox, ibox;        // ibox = 1 / box
   double ecut, ecut2, ecut_el, ecut_el2, kappa, lb, elj_att;
   double ks, kb, dt, damp, T;
-  double skin, half_skin2;
+  double skin, half_skin2;       // half_skin2 = (skin/2 - margin)^2, margin covering FP32 xb
   double sub_rc_lj2, sub_rc_q2;  // candidate-subunit centre distance^2 for a list build
   double rc_q_list2;             // (ecut_el + skin)^2
   double r_bound2;               // largest allowed bead distance^2 from its centre bead
@@ -168,7 +168,7 @@ struct Dev {
   double* vx; double* vy; double* vz;
   double* fx; double* fy; double* fz;  // integrator force (tforce)
   double* gx; double* gy; double* gz;  // BD drag force
-  double4* xb;         // positions at the last list build
+  float4* xb;          // positions at the last list build (FP32; the displacement test keeps a margin)
   double4* centres;    // [Ns] centre-bead positions
   int* lj_list; int* lj_cnt;
   int* q_list; int* q_cnt;
