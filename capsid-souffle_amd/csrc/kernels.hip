@@ -46,6 +46,24 @@ template <bool B>
 struct Flag {
   static constexpr bool value = B;
 };
+
+// Stores of bulk per-bead data that the following kernels read (positions, velocities, forces,
+// lists, cell records; not the positions of the last build, which every step reads back):
+// nontemporal, so they stream to memory instead of waiting in this XCD's L2
+// for the write-back at the kernel's end (the readers run on every XCD and read through memory
+// anyway). The step is ~2% shorter with them at 2000 subunits.
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) { __builtin_nontemporal_store(v, p); }
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void st_stream(double4* p, double4 v) {
+  dvec2* q = reinterpret_cast<dvec2*>(p);
+  __builtin_nontemporal_store((dvec2){v.x, v.y}, q);
+  __builtin_nontemporal_store((dvec2){v.z, v.w}, q + 1);
+}
+__device__ __forceinline__ void st_stream(float4* p, float4 v) {
+  __builtin_nontemporal_store((fvec4){v.x, v.y, v.z, v.w}, reinterpret_cast<fvec4*>(p));
+}
 // Branch-free single shift in FP32 for the list builds: the same value as the conditional shift
 // for a separation far below half a box; elsewhere both give a separation of about half a box,
 // so a test against a list radius (< half a box) decides the same.
@@ -193,14 +211,14 @@ __global__ __launch_bounds__(256) void k_integrate(Dev d) {
     vx = vx * e + fx * c; vy = vy * e + fy * c; vz = vz * e + fz * c;
   } else {
     const double cd = -1.0 * m / P.damp;
-    d.gx[b] = vx * cd; d.gy[b] = vy * cd; d.gz[b] = vz * cd;
+    st_stream(&d.gx[b], vx * cd); st_stream(&d.gy[b], vy * cd); st_stream(&d.gz[b], vz * cd);
     const double c = 0.5 * dt / m;
     vx = vx + fx * c; vy = vy + fy * c; vz = vz + fz * c;
   }
   p.x = p.x + vx * dt; p.y = p.y + vy * dt; p.z = p.z + vz * dt;
   p.x = min_img(p.x, box, hbox); p.y = min_img(p.y, box, hbox); p.z = min_img(p.z, box, hbox);  // one wrap
-  d.pos[b] = p;
-  d.vx[b] = vx; d.vy[b] = vy; d.vz[b] = vz;
+  st_stream(&d.pos[b], p);
+  st_stream(&d.vx[b], vx); st_stream(&d.vy[b], vy); st_stream(&d.vz[b], vz);
   const double ibox = P.ibox;
   const double ddx = min_img_near(p.x - (double)q.x, box, ibox), ddy = min_img_near(p.y - (double)q.y, box, ibox),
                ddz = min_img_near(p.z - (double)q.z, box, ibox);
@@ -538,7 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
   __syncthreads();
   const bool lj_fast = fallback_sh == 0;
   if (lj_fast && active) {
-    for (int k = 0; k < cnt_lj && k < d.lj_cap; k++) d.lj_list[(size_t)k * d.Nb + gi] = ljbuf[k * kBlock + tid];
+    for (int k = 0; k < cnt_lj && k < d.lj_cap; k++) st_stream(&d.lj_list[(size_t)k * d.Nb + gi], ljbuf[k * kBlock + tid]);
   }
   // ---- Yukawa: charged beads of the candidates, in candidate order
   bool q_fast = true;
@@ -594,7 +612,7 @@ __global__ __launch_bounds__(kBlock) void k_rebuild(Dev d, int force_rebuild) {
         const int gq = s * nq + qi;
         for (int f = 0; f < nrec; f++) {
           if ((ymask[f] >> qi) & 1u) {
-            if (cnt_q < d.q_cap) d.q_list[(size_t)cnt_q * d.Nq + gq] = __float_as_int(rec[f].w);
+            if (cnt_q < d.q_cap) st_stream(&d.q_list[(size_t)cnt_q * d.Nq + gq], __float_as_int(rec[f].w));
             cnt_q++;
           }
         }
@@ -682,7 +700,7 @@ __device__ __forceinline__ void fill_grid(bool in, int c, const float4& rec, int
   __syncthreads();
   if (in) {
     const int slot = base[h] + rank;
-    if (slot < cap) cells[(size_t)c * cap + slot] = rec;
+    if (slot < cap) st_stream(&cells[(size_t)c * cap + slot], rec);
   }
   __syncthreads();  // the tables are reused by the next grid
 }
@@ -1118,7 +1136,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
       }
     }
     if (!over) {
-      for (int k = 0; k < cnt_lj && k < d.lj_cap; k++) d.lj_list[(size_t)k * d.Nb + gi] = ljbuf[k * kBlock + tid];
+      for (int k = 0; k < cnt_lj && k < d.lj_cap; k++) st_stream(&d.lj_list[(size_t)k * d.Nb + gi], ljbuf[k * kBlock + tid]);
     } else {  // rare: the list went to global memory in pass 1; sort it there
       sort_column(d.lj_list + gi, d.Nb, min(cnt_lj, d.lj_cap));
     }
@@ -1195,7 +1213,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
         rank += (w.x < v) + (w.y < v) + (w.z < v) + (w.w < v);
       }
       for (int m = n & ~3; m < n; m++) rank += buf[m] < v ? 1 : 0;
-      if (rank < d.q_cap) d.q_list[(size_t)rank * d.Nq + s * nq + a] = v;
+      if (rank < d.q_cap) st_stream(&d.q_list[(size_t)rank * d.Nq + s * nq + a], v);
     }
     if (qi >= 0) {
       const int gq = s * nq + qi;
@@ -1679,8 +1697,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
       // second half kick (src/md.cpp:451-455)
       const double e = S.expfac, c = S.kick_c / m;
       vx = vx * e + Fx * c; vy = vy * e + Fy * c; vz = vz * e + Fz * c;
-      d.vx[gi] = vx; d.vy[gi] = vy; d.vz[gi] = vz;
-      d.fx[gi] = Fx; d.fy[gi] = Fy; d.fz[gi] = Fz;
+      st_stream(&d.vx[gi], vx); st_stream(&d.vy[gi], vy); st_stream(&d.vz[gi], vz);
+      st_stream(&d.fx[gi], Fx); st_stream(&d.fy[gi], Fy); st_stream(&d.fz[gi], Fz);
     } else {
       // BD: tforce = F + (fran - mean) + fdrag; v += tforce dt/(2m) (src/md.cpp:462-498)
       double ux, uy, uz;
@@ -1690,8 +1708,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
       Fx = Fx + rx + d.gx[gi]; Fy = Fy + ry + d.gy[gi]; Fz = Fz + rz + d.gz[gi];
       const double c = 0.5 * dt / m;
       vx = vx + Fx * c; vy = vy + Fy * c; vz = vz + Fz * c;
-      d.vx[gi] = vx; d.vy[gi] = vy; d.vz[gi] = vz;
-      d.fx[gi] = Fx; d.fy[gi] = Fy; d.fz[gi] = Fz;
+      st_stream(&d.vx[gi], vx); st_stream(&d.vy[gi], vy); st_stream(&d.vz[gi], vz);
+      st_stream(&d.fx[gi], Fx); st_stream(&d.fy[gi], Fy); st_stream(&d.fz[gi], Fz);
     }
     part[0] = 0.5 * m * (vx * vx + vy * vy + vz * vz);
     part[1] = vx; part[2] = vy; part[3] = vz;
