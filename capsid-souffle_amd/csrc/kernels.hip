@@ -1737,7 +1737,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
 
 // One workgroup per replica: fixed-order sums of the per-subunit partials, then the scalar
 // bookkeeping of src/md.cpp:400-406, 456-460, 469-478, 506-514.
-constexpr int kFinThreads = 1024;
+constexpr int kFinThreads = 512;
 
 __global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, int init) {
   __shared__ double wred[kFinThreads / 64][kPartSlots];
