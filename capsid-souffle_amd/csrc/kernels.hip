@@ -779,6 +779,14 @@ struct LjTest {
                 dz = min_imgf_near(iz - e.z, box, ibox);
     return dx * dx + dy * dy + dz * dz;
   }
+  // the same for a record known to lie less than half a box away on every axis, where the shift
+  // of min_imgf_near rounds to zero: the same value
+  template <bool WRAP>
+  __device__ __forceinline__ float r2(const float4& e) const {
+    if (WRAP) return r2_to(e);
+    const float dx = ix - e.x, dy = iy - e.y, dz = iz - e.z;
+    return dx * dx + dy * dy + dz * dz;
+  }
   __device__ __forceinline__ bool hit(float r2, int tj, float sj) const {
     if ((mask_i >> tj) & 1ull) return r2 < ratt2;
     const float w = (float)kWcaFactor * 0.5f * (si + sj) + skinf;
@@ -1080,7 +1088,15 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
         hi[k] = !in || 1.0f - f < rr * ia;
       }
     }
+    // no record needs an image shift when the local box neither wraps around the periodic grid
+    // nor spans half of it on any axis
+    bool wrap = g == 1;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      wrap = wrap || ax[k].full || ax[k].cc + ax[k].lo < 0 || ax[k].cc + ax[k].lo + ax[k].span > g || 2 * ax[k].span >= g;
     bool over = false;
+    auto scan = [&](auto wrap_tag) {
+    constexpr bool W = decltype(wrap_tag)::value;
     for (int pass = 0; pass < 2; pass++) {  // pass 1 only when the LDS slots overflowed
       if (pass == 1) {
         if (!over) break;
@@ -1113,7 +1129,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
           for (int u = 0; u < 4; u++) {
             if (m + u >= m1) break;
             // distance first; the pair's type and sigma only for records inside the bound
-            const float r2 = T.r2_to(e[u]);
+            const float r2 = T.r2<W>(e[u]);
             if (!(r2 < bound2)) continue;
             const int j = __float_as_int(e[u].w);
             const float2 tj = tts[local_of(j, np, inv_np)];
@@ -1135,6 +1151,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
         }
       }
     }
+    };
+    if (wrap) scan(Flag<true>{}); else scan(Flag<false>{});
     if (!over) {
       for (int k = 0; k < cnt_lj && k < d.lj_cap; k++) st_stream(&d.lj_list[(size_t)k * d.Nb + gi], ljbuf[k * kBlock + tid]);
     } else {  // rare: the list went to global memory in pass 1; sort it there

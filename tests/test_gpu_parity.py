@@ -60,6 +60,31 @@ def test_forces_match_oracle(template, seed, spacing):
         assert np.linalg.norm(ref[3]) > 0
 
 
+def test_forces_under_translation_and_outside_the_box():
+    """One periodic state in three representations: as placed (subunits around the origin), moved
+    by half a box along x and wrapped into the box (subunits straddling the x faces), and moved
+    without wrapping (coordinates up to ~14 sigma outside the box). Forces of all three agree with
+    the oracle's at 1e-10. The force kernel takes plain coordinate differences only where they equal
+    the minimum image (a subunit inside a quarter box of its centre bead; beads away from the faces
+    when every position was given inside the box), so these cover its shifted and unshifted paths."""
+    t, p, sysg = _system(20)
+    pos, vel = _state(TEMPLATE, 20, p.box, 11, 8.3)
+    h = p.box / 2
+    o = _oracle(20)
+    o.set_state(pos, vel)
+    ref = o.forces()
+    moved = pos + np.array([h, 0.0, 0.0])
+    wrapped = np.where(moved > h, moved - p.box, moved)
+    assert (np.abs(moved) > h).sum() > 100
+    assert (wrapped[:, 0] > h - 2).any() and (wrapped[:, 0] < -h + 2).any()
+    for x in (pos, wrapped, moved):
+        sysg.set_state(x, vel)
+        got = sysg.forces()
+        for name, g, r in zip(["total", "stretch", "bend", "yukawa", "lj"], got, ref):
+            assert rel_norm(g, r) < 1e-10, (name, rel_norm(g, r))
+    assert np.linalg.norm(ref[4]) > 0 and np.linalg.norm(ref[3]) > 0
+
+
 def test_energies_match_oracle():
     t, p, sysg = _system(20)
     pos, vel = _state(TEMPLATE, 20, p.box, 4, 8.6)
