@@ -81,16 +81,18 @@ def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
         s.set_timing(True)
     e_start = extended_energy(s)
     tune = None
+    s.set_graph_steps(args.graph_steps)
     s.step(args.warmup)
     if args.tune_tile:
         # the force kernel's pair tile by the library's autotuner at the fixed skin, after the
         # warm-up so that the trials see clusters forming (trials run from a saved state that is
         # restored; forces do not depend on the tile)
         tune, _ = s.autotune(skins=[s.skin(0)], tiles=[128, 256, 384, 512], trial_steps=200)
-    # one untimed pass with the timed run's chunk lengths (100-step chunks and the remainder):
-    # graph capture for those lengths and the list build the tuner's last setting requests happen
-    # here, not in the timed window; its rate is reported as the first-window figure
-    n_pre = args.steps % 100 + (100 if args.steps >= 100 else 0)
+    # one untimed pass with the timed run's launch lengths (graph_steps-step launches and the
+    # remainder): graph capture for those lengths and the list build the tuner's last setting
+    # requests happen here, not in the timed window; its rate is reported as the first-window figure
+    g = s.stats()["graph_steps"]
+    n_pre = args.steps % g + (g if args.steps >= g else 0)
     t0 = time.perf_counter()
     s.step(n_pre)
     first = time.perf_counter() - t0
@@ -103,6 +105,7 @@ def run_gpu(args, brownian=False, fp32=False, skin=None, timed_pass=True):
     nb = s.n_beads
     out = dict(nb=nb, wall=wall, value=nb * args.steps / wall, ms_per_step=1e3 * wall / args.steps,
                rebuilds=st1["rebuilds"] - st0["rebuilds"],
+               graph=dict(steps_per_launch=g, launches_timed=st1["graph_launches"] - st0["graph_launches"]),
                n_list=(st1["lj_entries"] + st1["q_entries"]) / nb,
                first_window=dict(steps=n_pre, bead_steps_per_s=nb * n_pre / first, ms_per_step=1e3 * first / n_pre),
                steps_run=args.warmup + n_pre + args.steps)
@@ -216,6 +219,9 @@ def main():
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--rep-steps", type=int, default=500, help="timed steps of the replica summary in the md line")
     ap.add_argument("--tune-steps", type=int, default=100)
+    ap.add_argument("--graph-steps", type=int, default=100, help="steps per graph launch of the md workload")
+    ap.add_argument("--rep-one-launch", type=int, default=1,
+                    help="replica batch: the timed steps as one graph launch (0: 100-step launches)")
     ap.add_argument("--cpu-replicas", type=int, default=8)
     ap.add_argument("--cpu-replica-steps", type=int, default=400)
     ap.add_argument("--subunits", type=int, default=2000)
@@ -314,8 +320,12 @@ def bench_replicas(args):
     skins = getattr(args, "tune_skins", [0.2, 0.3, 0.5, 0.8, 1.2])
     tiles = getattr(args, "tune_tiles", [128, 256, 384, 512])
     tune, trial_ms = batch.autotune(skins=skins, tiles=tiles, trial_steps=args.tune_steps)
-    # untimed pass with the timed run's chunk lengths (graph capture, the tuner's list build)
-    n_pre = args.steps % 100 + (100 if args.steps >= 100 else 0)
+    # the timed steps as one graph launch (no host round trip inside the timed window) when they
+    # fit one graph; an untimed pass of the same length captures it first (and runs the list
+    # build the tuner's last setting requests)
+    g = min(args.steps, 20000) if args.rep_one_launch else 100
+    batch.set_graph_steps(g)
+    n_pre = args.steps % g + (g if args.steps >= g else 0)
     batch.step(n_pre)
     batch.step(args.warmup)
     st0 = batch.stats()
@@ -361,6 +371,7 @@ def bench_replicas(args):
         "autotune": {"skin": tune["skin"], "tile": tune["tile"], "trial_ms_per_step": trial_ms.round(4).tolist(),
                      "skins": skins, "tiles": tiles},
         "single_run_check": checked,
+        "graph": {"steps_per_launch": g, "launches_timed": st1["graph_launches"] - st0["graph_launches"]},
     }
     if not args.no_cpu:
         from oracle_lib import Oracle  # the CPU leg only
@@ -550,7 +561,8 @@ def replica_summary(args):
     a.tune_skins, a.tune_tiles = [0.3, 0.5, 0.8], [256, 384, 512]
     a.cpu_replicas, a.cpu_replica_steps = 4, 100
     r = bench_replicas(a)
-    keep = {k: r[k] for k in ("value", "ms_per_step", "steps", "warmup", "autotune", "single_run_check", "kernels_ms")}
+    keep = {k: r[k] for k in ("value", "ms_per_step", "steps", "warmup", "autotune", "single_run_check", "kernels_ms",
+                              "graph")}
     keep["workload"] = r["config"]["workload"]
     keep["beads"] = r["config"]["beads"]
     keep["roofline_frac"] = r["roofline"]["frac"]
@@ -594,6 +606,7 @@ def bench_md(args):
         # the untimed pass before the timed window (graph capture of its chunk lengths, the tuner's
         # list build)
         "first_window": md["first_window"],
+        "graph": md["graph"],
     }
     if not args.no_extra:
         bd = run_gpu(args, brownian=True, timed_pass=False)

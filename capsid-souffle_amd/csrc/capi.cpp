@@ -26,7 +26,8 @@ namespace {
     }                                                                                  \
   } while (0)
 
-constexpr int kChunk = 100;          // steps between capacity checks
+constexpr int kChunk = 100;          // steps between capacity checks (default graph length)
+constexpr int kMaxGraphSteps = 20000;  // longest graph a handle captures (5 kernel nodes a step)
 constexpr int kTimeEvery = 10;       // with timing on, every 10th step's kernels get HIP events
 constexpr double kBoundMargin = 1.0; // initial slack of the centre-bead radius (sigma)
 constexpr double kNoCut2 = 1e30;     // squared "cutoff" of the uncut Yukawa's bead tests
@@ -70,6 +71,10 @@ struct sf_handle {
   int mode = MODE_MD;
   int fp32 = 0;
   bool timing = false;
+  // steps per graph launch between host checks of the capacity, deformation and finiteness
+  // flags (sf_set_graph_steps); launches counted since sf_create
+  int graph_steps = kChunk;
+  long long graph_launches = 0;
   double t_int = 0, t_rb = 0, t_force = 0, t_red = 0, t_steps = 0, t_far = 0;
   // snapshot for replaying a chunk
   double4* s_pos = nullptr; double* s_v = nullptr; double* s_f = nullptr; double* s_g = nullptr;
@@ -484,6 +489,7 @@ struct sf_handle {
       graphs.emplace_back(n, ex);
     }
     HIPCHK(hipGraphLaunch(ex, st));
+    graph_launches++;
     return SF_OK;
   }
 };
@@ -919,8 +925,11 @@ int sf_step(sf_handle* h, int n_steps) {
   bool nonfinite = false;
   long long nf_step = 0;
   int nf_rep = -1;
+  // a far-field model is audited every kChunk steps and timed steps launch directly, so both
+  // keep the default length; otherwise one graph launch covers up to graph_steps steps
+  const int chunk = (h->far_model || h->timing) ? kChunk : h->graph_steps;
   while (done < goal) {
-    const int n = std::min(kChunk, goal - done);
+    const int n = std::min(chunk, goal - done);
     if ((rc = h->snapshot(true))) break;
     if ((rc = h->launch_chunk(n))) break;
     rc = h->check_and_fix();
@@ -1005,6 +1014,8 @@ int sf_get_stats(sf_handle* h, sf_stats* o) {
   o->n_beads = h->Nb; o->n_subunits = h->Ns; o->n_charged = h->Nq; o->n_replicas = h->R;
   o->k_integrate_ms = h->t_int; o->k_rebuild_ms = h->t_rb; o->k_force_ms = h->t_force; o->k_reduce_ms = h->t_red;
   o->k_timed_steps = h->t_steps;
+  o->graph_launches = h->graph_launches;
+  o->graph_steps = h->graph_steps;
   o->k_far_ms = h->t_far;
   o->vcom = s[0].vcom;
   return SF_OK;
@@ -1032,6 +1043,13 @@ int sf_bd_uniforms(sf_handle* h, int replica, int64_t step, double* out) {
   }
   h->release(dout);
   return rc;
+}
+
+int sf_set_graph_steps(sf_handle* h, int steps) {
+  if (!h) { set_error("null handle"); return SF_E_ARG; }
+  if (steps > kMaxGraphSteps) { set_error("graph length above 20000 steps"); return SF_E_ARG; }
+  h->graph_steps = steps <= 0 ? kChunk : steps;
+  return SF_OK;
 }
 
 int sf_set_tile(sf_handle* h, int tile) {

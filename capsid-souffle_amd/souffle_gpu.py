@@ -38,7 +38,8 @@ class SfStats(ctypes.Structure):
                 ("n_replicas", ctypes.c_int32), ("k_integrate_ms", ctypes.c_double),
                 ("k_rebuild_ms", ctypes.c_double),
                 ("k_force_ms", ctypes.c_double), ("k_reduce_ms", ctypes.c_double),
-                ("k_timed_steps", ctypes.c_double), ("vcom", ctypes.c_double), ("k_far_ms", ctypes.c_double)]
+                ("k_timed_steps", ctypes.c_double), ("vcom", ctypes.c_double), ("k_far_ms", ctypes.c_double),
+                ("graph_launches", ctypes.c_int64), ("graph_steps", ctypes.c_int32)]
 
 
 class SfResume(ctypes.Structure):
@@ -100,6 +101,7 @@ SIGNATURES = {
     "sf_energies": (ctypes.c_int, [_P, _D, _D]),
     "sf_step": (ctypes.c_int, [_P, ctypes.c_int]),
     "sf_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sf_set_graph_steps": (ctypes.c_int, [_P, ctypes.c_int]),
     "sf_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(SfStats)]),
     "sf_bd_uniforms": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64, _D]),
     "sf_set_skin": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double]),
@@ -311,6 +313,10 @@ class System:
 
     def set_timing(self, on):
         _err(_lib.sf_set_timing(self.ptr, 1 if on else 0), "sf_set_timing")
+
+    def set_graph_steps(self, steps):
+        """Steps per HIP graph launch of step() (0: the default 100; at most 20000)."""
+        _err(_lib.sf_set_graph_steps(self.ptr, int(steps)), "sf_set_graph_steps")
 
     def stats(self):
         s = SfStats()

@@ -93,6 +93,8 @@ typedef struct {
   double k_timed_steps;   /* steps the means were taken over (0 if timing was off) */
   double vcom;            /* replica 0: |centre-of-mass velocity| removed at the last step */
   double k_far_ms;        /* far field of the uncut Yukawa (exact or model), as the means above */
+  int64_t graph_launches; /* HIP graph launches of the step loop since sf_create */
+  int32_t graph_steps;    /* steps per graph launch (sf_set_graph_steps) */
 } sf_stats;
 
 /* ---- template and parameters ---------------------------------------------------------- */
@@ -209,6 +211,16 @@ int sf_energies(sf_handle* h, double* totals, double* per_bead);
  * is left at the last finite step and SF_E_NONFINITE is returned; sf_last_error names the step
  * and the replica, and sf_get_state / sf_get_resume give that state. */
 int sf_step(sf_handle* h, int n_steps);
+
+/* Steps per HIP graph launch of sf_step (no reference counterpart: the reference's loop is host
+ * code, src/md.cpp:387-514). sf_step captures that many loop iterations of every replica into
+ * one graph (5 kernel nodes a step), launches it once, and reads the capacity, deformation and
+ * finiteness flags only when it ends; a launch that overflowed a capacity is replayed from its
+ * start with grown buffers, one that went non-finite up to its last finite step. 0 = the
+ * default of 100; at most 20000. A run of n <= steps iterations is then a single launch with no
+ * host round trip. With a far-field model attached (audited every 100 steps) or with timing on,
+ * launches stay 100 steps long. Results are the same bit for bit for every length. */
+int sf_set_graph_steps(sf_handle* h, int steps);
 
 /* Per-kernel timing with HIP events on the library's stream during sf_step (off by default). */
 int sf_set_timing(sf_handle* h, int enable);

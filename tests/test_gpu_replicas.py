@@ -92,3 +92,73 @@ def test_sweep_batch_matches_single_runs():
         xs, _ = single.thermostat(0)
         assert np.array_equal(xb, xs), k
         off += n
+
+
+def test_whole_run_in_one_graph_launch():
+    """A sweep batch run as one HIP graph launch of all its steps (no host round trip) against the
+    same batch in 100-step launches: the same trajectories bit for bit."""
+    t, sizes, params, pos, vel = sweep(12, 17)
+    x0, v0 = np.concatenate(pos), np.concatenate(vel)
+    a = sg.System(t, sizes, params)
+    a.set_state(x0, v0)
+    a.step(350)
+    st_a = a.stats()
+    ap, av = a.get_state()
+    b = sg.System(t, sizes, params)
+    b.set_state(x0, v0)
+    b.set_graph_steps(350)
+    n0 = b.stats()["graph_launches"]
+    b.step(350)
+    st_b = b.stats()
+    bp, bv = b.get_state()
+    assert st_b["graph_launches"] - n0 == 1 and st_b["graph_steps"] == 350
+    assert st_a["graph_steps"] == 100
+    assert np.array_equal(ap, bp) and np.array_equal(av, bv)
+    assert st_a["steps"] == st_b["steps"] == 350 and st_a["rebuilds"] == st_b["rebuilds"]
+    for k in range(len(sizes)):
+        assert np.array_equal(a.thermostat(k)[0], b.thermostat(k)[0]), k
+    with pytest.raises(sg.SfError):
+        b.set_graph_steps(20001)
+
+
+def test_long_graph_replays_after_list_growth():
+    """A larger skin set before a 150-step launch makes its first list build outgrow the
+    capacities: the launch is replayed from its start with grown lists, and the run equals the
+    one in 100-step launches."""
+    from souffle_states import make_state, read_template_xyz
+    t = sg.Template(TEMPLATE)
+    p = t.params(C_uM=800.0, c_mM=100.0, elj_att=2.1, dt=0.005, K=298.0, n_subunits=27)
+    pos, vel = make_state(read_template_xyz(TEMPLATE)[0], 27, p.box, 11, spacing=7.6, perturb=0.02)
+    runs = []
+    for steps in (0, 150):
+        s = sg.System(t, 27, p)
+        s.set_graph_steps(steps)
+        s.set_state(pos, vel)
+        s.set_skin(2.5)
+        st0 = s.stats()
+        s.step(150)
+        runs.append((s.get_state(), st0, s.stats()))
+    (ap, av), _, sa = runs[0]
+    (bp, bv), b0, sb = runs[1]
+    print("capacities", b0["lj_capacity"], b0["q_capacity"], "->", sb["lj_capacity"], sb["q_capacity"],
+          "launches", sb["graph_launches"] - b0["graph_launches"], flush=True)
+    assert np.array_equal(ap, bp) and np.array_equal(av, bv)
+    assert sb["lj_capacity"] == sa["lj_capacity"] and sb["q_capacity"] == sa["q_capacity"]
+    assert (sb["lj_capacity"], sb["q_capacity"]) != (b0["lj_capacity"], b0["q_capacity"])  # the lists grew
+    assert sb["graph_launches"] - b0["graph_launches"] >= 2  # the overflowed launch and its replay
+
+
+def test_long_graph_stops_at_the_last_finite_step(tmp_path):
+    """A run that goes non-finite inside one 50-step launch is replayed up to its last finite step."""
+    from template_writer import write_template
+    path = write_template(str(tmp_path / "point"), beads=[(0.0, 0.0, 0.0, 1, 0.0, 0.01, 1.0)])
+    t = sg.Template(path)
+    p = t.params(n_subunits=2, C_uM=50.0, c_mM=100.0, elj_att=2.1, dt=0.005, K=298.0, q_chain=1)
+    s = sg.System(t, 2, p)
+    s.set_graph_steps(50)
+    s.set_state(np.array([[-5.0, 0.0, 0.0], [5.0, 0.0, 0.0]]), np.array([[100.0, 0.0, 0.0], [-100.0, 0.0, 0.0]]))
+    with pytest.raises(sg.SfError) as e:
+        s.step(50)
+    assert e.value.code == sg.SF_E_NONFINITE
+    gp, _ = s.get_state()
+    assert np.array_equal(gp[:, 0], [-0.5, 0.5]) and s.stats()["steps"] == 9
