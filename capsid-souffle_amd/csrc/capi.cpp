@@ -676,6 +676,23 @@ sf_handle* sf_create(const sf_template* t, int R, const int* n_subunits, const s
   d.vx = vbuf; d.vy = vbuf + nb; d.vz = vbuf + 2 * nb;
   d.fx = fbuf; d.fy = fbuf + nb; d.fz = fbuf + 2 * nb;
   d.gx = gbuf; d.gy = gbuf + nb; d.gz = gbuf + 2 * nb;
+  d.bd_skip = nullptr;
+  if (brown) {  // skip-ahead multipliers of the BD engine, one per replica-local bead
+    int max_nb = 0;
+    for (int r = 0; r < R; r++) max_nb = std::max(max_nb, h->rp[r].nb);
+    std::vector<uint32_t> skip(max_nb);
+    const uint64_t m6 = lcg_pow(6);
+    uint64_t m = 1;
+    for (int b = 0; b < max_nb; b++) {
+      skip[b] = (uint32_t)m;
+      m = (m * m6) % kLcgMod;
+    }
+    uint32_t* sk;
+    if (h->alloc(&sk, max_nb) ||
+        hipMemcpy(sk, skip.data(), max_nb * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(nullptr);
+    d.bd_skip = sk;
+  }
   d.lj_cap = 0; d.q_cap = 0;
   if (h->grow_lists(24, nq > 0 ? 48 : 1)) return fail(nullptr);
   d.cells = (!split && (long long)h->Nb < (1ll << 23)) ? 1 : 0;

@@ -114,16 +114,20 @@ __device__ __forceinline__ double canonical_uniform(uint32_t g1, uint32_t g2) {
   return ret * 1.0 + (-0.5);
 }
 
-// The 3 uniform draws of bead b (replica-local) for the step whose engine state is `state`:
-// draws 6b+1 .. 6b+6 of the stream, x, y, z order (src/md.cpp:464-466).
-__device__ __forceinline__ void bd_draws(uint32_t state, int b, double& ux, double& uy, double& uz) {
-  uint32_t x = lcg_mulmod(state, lcg_powmod(6ull * (unsigned long long)b));
+// The 3 uniform draws of a bead for the step whose engine state is `state`: draws 6b+1 .. 6b+6
+// of the stream (b replica-local), x, y, z order (src/md.cpp:464-466); skip = 16807^(6b) mod
+// (2^31 - 1).
+__device__ __forceinline__ void bd_draws_skip(uint32_t state, uint32_t skip, double& ux, double& uy, double& uz) {
+  uint32_t x = lcg_mulmod(state, skip);
   uint32_t g[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) { x = lcg_mulmod(x, kLcgMul); g[k] = x; }
   ux = canonical_uniform(g[0], g[1]);
   uy = canonical_uniform(g[2], g[3]);
   uz = canonical_uniform(g[4], g[5]);
+}
+__device__ __forceinline__ void bd_draws(uint32_t state, int b, double& ux, double& uy, double& uz) {
+  bd_draws_skip(state, lcg_powmod(6ull * (unsigned long long)b), ux, uy, uz);
 }
 
 // Replica of subunit s; a handle holding one system skips the table lookup, so that the
@@ -1529,6 +1533,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
   const int4 hg1 = reinterpret_cast<const int4*>(d.topo.hinge)[tid + kBlock < nh ? tid + kBlock : 0];
   double vx = 0, vy = 0, vz = 0;
   if (active) { vx = d.vx[gi]; vy = d.vy[gi]; vz = d.vz[gi]; }
+  const uint32_t skip = (d.bd_skip && active) ? d.bd_skip[gi - P.b0] : 1u;  // BD engine skip-ahead
   // offsets of each bead's entries in the subunit's concatenated lists (LJ entries, then
   // Yukawa entries, bead by bead): wave prefix sums, then the two waves in order
   const int n_i = nlj + nqe;
@@ -1720,7 +1725,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
     } else {
       // BD: tforce = F + (fran - mean) + fdrag; v += tforce dt/(2m) (src/md.cpp:462-498)
       double ux, uy, uz;
-      bd_draws(S.rng, gi - P.b0, ux, uy, uz);
+      bd_draws_skip(S.rng, skip, ux, uy, uz);
       const double amp = P.bd_amp[tid];
       const double rx = amp * ux - S.mean_fran[0], ry = amp * uy - S.mean_fran[1], rz = amp * uz - S.mean_fran[2];
       Fx = Fx + rx + d.gx[gi]; Fy = Fy + ry + d.gy[gi]; Fz = Fz + rz + d.gz[gi];
@@ -1736,7 +1741,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) 
       // given, the seed for a fresh start) for its mean
       const uint32_t nxt = init ? S.rng : lcg_mulmod(S.rng, P.mult6n);
       double ux, uy, uz;
-      bd_draws(nxt, gi - P.b0, ux, uy, uz);
+      bd_draws_skip(nxt, skip, ux, uy, uz);
       const double amp = P.bd_amp[tid];
       part[4] = amp * ux; part[5] = amp * uy; part[6] = amp * uz;
     }

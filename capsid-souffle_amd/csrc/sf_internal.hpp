@@ -193,6 +193,9 @@ struct Dev {
   // ecut_el^2, shc^6), and, for the energy entries, the big-bead terms of each subunit bead
   // [2 Nb] = (ljforce, ne), (eforce, ce)
   int nbig;
+  // BD noise: 16807^(6 b) mod (2^31 - 1) for replica-local bead b, the engine's skip-ahead to
+  // the bead's first draw of a step (null for MD handles)
+  const uint32_t* bd_skip;
   const double4* big_pos;
   const double4* big_tab;
   double4* big_f;
