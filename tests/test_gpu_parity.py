@@ -85,10 +85,13 @@ def test_forces_under_translation_and_outside_the_box():
     assert np.linalg.norm(ref[4]) > 0 and np.linalg.norm(ref[3]) > 0
 
 
-def test_energies_match_oracle():
-    t, p, sysg = _system(20)
-    pos, vel = _state(TEMPLATE, 20, p.box, 4, 8.6)
-    o = _oracle(20)
+@pytest.mark.parametrize("template,seed,spacing", [(TEMPLATE, 4, 8.6), (TEMPLATE_BIN, 3, 8.2)])
+def test_energies_match_oracle(template, seed, spacing):
+    """Energy totals and per-bead terms; both copies of the template (the bin/infiles one has
+    sigma = 0.8889 nm and no charges: its Yukawa terms are exactly zero on both sides)."""
+    t, p, sysg = _system(20, template)
+    pos, vel = _state(template, 20, p.box, seed, spacing)
+    o = _oracle(20, template)
     o.set_state(pos, vel)
     ref, ref_pb = o.energies(per_bead=True)
     sysg.set_state(pos, vel)
@@ -98,14 +101,20 @@ def test_energies_match_oracle():
         assert abs(got[k] - ref[k]) <= 1e-10 * max(abs(ref[k]), 1e-300), (name, got[k], ref[k])
     # per bead: se, be, ne, ce, ke in the reference's BEAD fields
     for k in range(5):
+        if np.linalg.norm(ref_pb[:, k]) == 0:
+            assert np.linalg.norm(got_pb[:, k]) == 0, k
+            continue
         assert rel_norm(got_pb[:, k], ref_pb[:, k]) < 1e-10, k
+    assert ref[3] != 0, "state has no LJ contacts"
+    if template == TEMPLATE_BIN:
+        assert ref[4] == 0 and got[4] == 0
 
 
-@pytest.mark.parametrize("q_chain", [5, 1])
-def test_md_trajectory_matches_oracle(q_chain):
-    t, p, sysg = _system(20, q_chain=q_chain)
-    pos, vel = _state(TEMPLATE, 20, p.box, 5, 8.6)
-    o = _oracle(20, q_chain=q_chain, L=3.0, B=5)
+@pytest.mark.parametrize("q_chain,template,spacing", [(5, TEMPLATE, 8.6), (1, TEMPLATE, 8.6), (5, TEMPLATE_BIN, 8.2)])
+def test_md_trajectory_matches_oracle(q_chain, template, spacing):
+    t, p, sysg = _system(20, template, q_chain=q_chain)
+    pos, vel = _state(template, 20, p.box, 5, spacing)
+    o = _oracle(20, template, q_chain=q_chain, L=3.0, B=5)
     o.set_state(pos, vel)
     sysg.set_state(pos, vel)
     n = 200
@@ -133,10 +142,11 @@ def test_bd_noise_stream_bit_exact():
         assert np.array_equal(got, ref), step
 
 
-def test_bd_trajectory_matches_oracle():
-    t, p, sysg = _system(20, brownian=True)
-    pos, vel = _state(TEMPLATE, 20, p.box, 6, 8.6)
-    o = _oracle(20, brownian=True, L=3.0, B=5)
+@pytest.mark.parametrize("template,spacing", [(TEMPLATE, 8.6), (TEMPLATE_BIN, 8.2)])
+def test_bd_trajectory_matches_oracle(template, spacing):
+    t, p, sysg = _system(20, template, brownian=True)
+    pos, vel = _state(template, 20, p.box, 6, spacing)
+    o = _oracle(20, template, brownian=True, L=3.0, B=5)
     o.set_state(pos, vel)
     sysg.set_state(pos, vel)
     n = 100
