@@ -78,6 +78,15 @@ __device__ __forceinline__ int local_of(int j, int np, float inv_np) {
   return r;
 }
 
+// The same for j < 2^23 (the cell builds' range): the float quotient is then within 1/np of
+// j / np, so one correction step each way is exact (no loop).
+__device__ __forceinline__ int local_of_small(int j, int np, float inv_np) {
+  const int q = (int)((float)j * inv_np);
+  int r = j - q * np;
+  r = r < 0 ? r + np : r;
+  return r >= np ? r - np : r;
+}
+
 __device__ __forceinline__ int tag_sub(double w) { return (int)(__double_as_longlong(w) >> 32); }
 __device__ __forceinline__ int tag_loc(double w) { return (int)(__double_as_longlong(w) & 0xffffffffll); }
 
@@ -811,19 +820,34 @@ struct CellAxis {
   bool full;
   __device__ __forceinline__ int local(int rel) const { return full ? wrap_cell(cc + rel, g) : rel - lo; }
   __device__ __forceinline__ int global(int l) const { return full ? l : wrap_cell(cc + lo + l, g); }
+  // periodic image of local cell l (or of relative cell rel) in the workgroup's unwrapped frame:
+  // 1 = one box below the stored position, 2 = one box above, 0 = as stored (always for full axes)
+  __device__ __forceinline__ int shift_code(int l) const {
+    if (full) return 0;
+    const int u = cc + lo + l;
+    return u < 0 ? 1 : (u >= g ? 2 : 0);
+  }
+  __device__ __forceinline__ float shift_rel(int rel, float box) const {
+    if (full) return 0.0f;
+    const int u = cc + rel;
+    return u < 0 ? -box : (u >= g ? box : 0.0f);
+  }
 };
+__device__ __forceinline__ float shift_of(int code, float box) { return code == 1 ? -box : (code == 2 ? box : 0.0f); }
 
 // The cells of a g^3 grid (first cell c0) that the 27-cell stencils of a workgroup's
 // participating beads touch are marked, and the records of other subunits in them (the cell's
 // records less the workgroup's own beads, ids in [excl_lo, excl_hi), whose cells the
 // participating beads count themselves) are copied into LDS packed in local-cell order: cell lc
-// holds rec[coff[lc] .. coff[lc] + cnum[lc]), and consecutive cells are contiguous. Each thread
-// owns four consecutive local cells for the counting, the prefix sum and the copy. Returns false
-// (uniformly) when the box or the records do not fit.
+// holds rec[coff[lc] .. coff[lc] + cnum[lc]), and consecutive cells are contiguous. Records are
+// moved to their periodic image in the workgroup's unwrapped frame around the centre cell (along
+// axes the local box does not cover whole), so that beads moved into the same frame take plain
+// differences. Each thread owns four consecutive local cells for the counting, the prefix sum and
+// the copy. Returns false (uniformly) when the box or the records do not fit.
 __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt, int cap, int c0, int g, float inv_a, float hbox,
                              float cx, float cy, float cz, bool part, float px, float py, float pz, int tid,
                              CellAxis* ax, int* rel_out, int* cnum, int* ownc, int* coff, float4* rec, int* red_i,
-                             int excl_lo, int excl_hi) {
+                             int excl_lo, int excl_hi, float box) {
   const int ccx = cell_of(cx, hbox, inv_a, g), ccy = cell_of(cy, hbox, inv_a, g), ccz = cell_of(cz, hbox, inv_a, g);
   int rx = 0, ry = 0, rz = 0;
   if (part) {
@@ -882,12 +906,16 @@ __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt
   for (int k = 0; k < 4; k++) {
     const int lc = 4 * tid + k;
     int n = 0, sn = 0, gl = c0;
+    int code = 0;
     if (lc < total && cnum[lc]) {
-      gl = c0 + (ax[2].global(lc / (sx * sy)) * g + ax[1].global((lc / sx) % sy)) * g + ax[0].global(lc % sx);
+      const int lz = lc / (sx * sy), ly = (lc / sx) % sy, lx = lc % sx;
+      gl = c0 + (ax[2].global(lz) * g + ax[1].global(ly)) * g + ax[0].global(lx);
+      code = ax[0].shift_code(lx) | (ax[1].shift_code(ly) << 2) | (ax[2].shift_code(lz) << 4);
       sn = min(cnt[gl], cap);
       n = max(sn - ownc[lc], 0);
     }
-    loc[k] = n; src[k] = sn; gcell[k] = gl;
+    // global cell (< 2^24: grids hold at most two cells per bead, beads < 2^23) and image code
+    loc[k] = n; src[k] = sn; gcell[k] = gl | (code << 24);
     sum += n; ssum += sn;
   }
   int incl = sum, sincl = ssum;
@@ -930,9 +958,14 @@ __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt
       if (send[mid] > rr) hi = mid; else lo = mid + 1;
     }
     const int start = lo ? send[lo - 1] : 0;
-    const float4 v = cells[(size_t)gcl[lo] * cap + (rr - start)];
+    const int gc = gcl[lo];
+    float4 v = cells[(size_t)(gc & 0xffffff) * cap + (rr - start)];
     const int j = __float_as_int(v.w);
     if (j >= excl_lo && j < excl_hi) continue;
+    const int code = gc >> 24;  // < 64: the packed value stays positive
+    v.x += shift_of(code & 3, box);
+    v.y += shift_of((code >> 2) & 3, box);
+    v.z += shift_of(code >> 4, box);
     const int w = atomicAdd(&run[lo], 1);
     if (w < cnum[lo]) rec[coff[lo] + w] = v;
   }
@@ -971,7 +1004,7 @@ __device__ __forceinline__ void cell_query_direct(const Dev& d, const RepParams&
       for (int m = 0; m < n; m++) {
         const float4 e = cell[m];
         const int j = __float_as_int(e.w);
-        const int lj = local_of(j, np, inv_np);
+        const int lj = local_of_small(j, np, inv_np);
         if (j - lj == s * np || !T(e, __float_as_int(tts[lj].x), tts[lj].y)) continue;
         if (cnt_lj < d.lj_cap) d.lj_list[(size_t)cnt_lj * d.Nb + gi] = j;
         cnt_lj++;
@@ -1066,7 +1099,7 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   int rel[3];
   // ---- LJ / WCA: the neighbourhood's records of other subunits, gathered into LDS
   const bool lj_ok = gather_cells(d.cell_lj, d.cnt_lj, d.ccap_lj, P.c0_lj, P.g_lj, P.inv_a_lj, hbox, cx, cy, cz, active,
-                                  T.ix, T.iy, T.iz, tid, ax, rel, cnum, ownc, coff, rec, red_i, s * np, (s + 1) * np);
+                                  T.ix, T.iy, T.iz, tid, ax, rel, cnum, ownc, coff, rec, red_i, s * np, (s + 1) * np, box);
   bool direct = !lj_ok;
   if (lj_ok && active) {
     // the local box in registers; with no wrap along x the three cells of a (dz, dy) row are one
@@ -1092,12 +1125,15 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
         hi[k] = !in || 1.0f - f < rr * ia;
       }
     }
-    // no record needs an image shift when the local box neither wraps around the periodic grid
-    // nor spans half of it on any axis
+    // the bead in the records' unwrapped frame; no record then needs an image shift unless the
+    // local box covers an axis whole or spans half of it
+    LjTest TL = T;
+    TL.ix += ax[0].shift_rel(rel[0], box);
+    TL.iy += ax[1].shift_rel(rel[1], box);
+    TL.iz += ax[2].shift_rel(rel[2], box);
     bool wrap = g == 1;
 #pragma unroll
-    for (int k = 0; k < 3; k++)
-      wrap = wrap || ax[k].full || ax[k].cc + ax[k].lo < 0 || ax[k].cc + ax[k].lo + ax[k].span > g || 2 * ax[k].span >= g;
+    for (int k = 0; k < 3; k++) wrap = wrap || ax[k].full || 2 * ax[k].span >= g;
     bool over = false;
     auto scan = [&](auto wrap_tag) {
     constexpr bool W = decltype(wrap_tag)::value;
@@ -1133,11 +1169,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
           for (int u = 0; u < 4; u++) {
             if (m + u >= m1) break;
             // distance first; the pair's type and sigma only for records inside the bound
-            const float r2 = T.r2<W>(e[u]);
+            const float r2 = TL.r2<W>(e[u]);
             if (!(r2 < bound2)) continue;
             const int j = __float_as_int(e[u].w);
-            const float2 tj = tts[local_of(j, np, inv_np)];
-            if (!T.hit(r2, __float_as_int(tj.x), tj.y)) continue;
+            const float2 tj = tts[local_of_small(j, np, inv_np)];
+            if (!TL.hit(r2, __float_as_int(tj.x), tj.y)) continue;
             if (pass == 1) {
               if (cnt_lj < d.lj_cap) d.lj_list[(size_t)cnt_lj * d.Nb + gi] = j;
             } else if (cnt_lj < kCqLjSlots) {
