@@ -1163,17 +1163,24 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
         }
         for (int m = m0; m < m1; m += 4) {  // four records requested together
           float4 e[4];
+          float r2[4];
 #pragma unroll
           for (int u = 0; u < 4; u++) e[u] = rec[min(m + u, m1 - 1)];
+          // distances first, without branches; the pair's type and sigma only for the few records
+          // inside the bound, in record order
+          unsigned cand = 0;
 #pragma unroll
           for (int u = 0; u < 4; u++) {
-            if (m + u >= m1) break;
-            // distance first; the pair's type and sigma only for records inside the bound
-            const float r2 = TL.r2<W>(e[u]);
-            if (!(r2 < bound2)) continue;
+            r2[u] = TL.r2<W>(e[u]);
+            cand |= (m + u < m1 && r2[u] < bound2) ? 1u << u : 0u;
+          }
+          if (cand == 0) continue;
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            if (!((cand >> u) & 1u)) continue;
             const int j = __float_as_int(e[u].w);
             const float2 tj = tts[local_of_small(j, np, inv_np)];
-            if (!TL.hit(r2, __float_as_int(tj.x), tj.y)) continue;
+            if (!TL.hit(r2[u], __float_as_int(tj.x), tj.y)) continue;
             if (pass == 1) {
               if (cnt_lj < d.lj_cap) d.lj_list[(size_t)cnt_lj * d.Nb + gi] = j;
             } else if (cnt_lj < kCqLjSlots) {
