@@ -847,7 +847,7 @@ __device__ __forceinline__ float shift_of(int code, float box) { return code == 
 __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt, int cap, int c0, int g, float inv_a, float hbox,
                              float cx, float cy, float cz, bool part, float px, float py, float pz, int tid,
                              CellAxis* ax, int* rel_out, int* cnum, int* ownc, int* coff, float4* rec, int* red_i,
-                             int excl_lo, int excl_hi, float box) {
+                             int excl_lo, int excl_hi, float box, int reach) {
   const int ccx = cell_of(cx, hbox, inv_a, g), ccy = cell_of(cy, hbox, inv_a, g), ccz = cell_of(cz, hbox, inv_a, g);
   int rx = 0, ry = 0, rz = 0;
   if (part) {
@@ -883,17 +883,16 @@ __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt
   if (total > kCqCells) return false;  // uniform: every thread reads the same spans
   for (int lc = tid; lc < total; lc += kBlock) { cnum[lc] = 0; ownc[lc] = 0; }
   __syncthreads();
-  if (part) {  // mark the stencil cells (cnum used as a flag first) and count the own bead
-    for (int dz = -1; dz <= 1; dz++) {
-      if (g == 1 && dz != 0) continue;
+  if (part) {  // mark the stencil cells the bead reaches (cnum used as a flag first) and count the own bead
+    const bool one = g == 1;
+    const int z0 = (!one && (reach & 4)) ? -1 : 0, z1 = (!one && (reach & 32)) ? 1 : 0;
+    const int y0 = (!one && (reach & 2)) ? -1 : 0, y1 = (!one && (reach & 16)) ? 1 : 0;
+    const int x0 = (!one && (reach & 1)) ? -1 : 0, x1 = (!one && (reach & 8)) ? 1 : 0;
+    for (int dz = z0; dz <= z1; dz++) {
       const int lz = ax[2].local(rz + dz);
-      for (int dy = -1; dy <= 1; dy++) {
-        if (g == 1 && dy != 0) continue;
+      for (int dy = y0; dy <= y1; dy++) {
         const int ly = ax[1].local(ry + dy);
-        for (int dx = -1; dx <= 1; dx++) {
-          if (g == 1 && dx != 0) continue;
-          cnum[(lz * sy + ly) * sx + ax[0].local(rx + dx)] = 1;
-        }
+        for (int dx = x0; dx <= x1; dx++) cnum[(lz * sy + ly) * sx + ax[0].local(rx + dx)] = 1;
       }
     }
     atomicAdd(&ownc[(ax[2].local(rz) * sy + ax[1].local(ry)) * sx + ax[0].local(rx)], 1);
@@ -902,13 +901,18 @@ __device__ __forceinline__ bool gather_cells(const float4* cells, const int* cnt
   // records of other subunits per marked cell, then exclusive prefix sums of those (the packed
   // LDS layout) and of all stored records (the copy's work list)
   int loc[4], src[4], gcell[4], sum = 0, ssum = 0;
+  // local coordinates of the thread's first cell; the next three follow by counting
+  int lz = (4 * tid) / (sx * sy), ly = (4 * tid - lz * sx * sy) / sx, lx = 4 * tid - (lz * sy + ly) * sx;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int lc = 4 * tid + k;
     int n = 0, sn = 0, gl = c0;
     int code = 0;
+    if (k > 0 && ++lx == sx) {
+      lx = 0;
+      if (++ly == sy) { ly = 0; lz++; }
+    }
     if (lc < total && cnum[lc]) {
-      const int lz = lc / (sx * sy), ly = (lc / sx) % sy, lx = lc % sx;
       gl = c0 + (ax[2].global(lz) * g + ax[1].global(ly)) * g + ax[0].global(lx);
       code = ax[0].shift_code(lx) | (ax[1].shift_code(ly) << 2) | (ax[2].shift_code(lz) << 4);
       sn = min(cnt[gl], cap);
@@ -1097,9 +1101,27 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
   const float bound2 = T.bound2(fmaxf(smax_w[0], smax_w[1]));
   int cnt_lj = 0, cnt_q = 0;
   int rel[3];
-  // ---- LJ / WCA: the neighbourhood's records of other subunits, gathered into LDS
+  // ---- LJ / WCA: the neighbourhood's records of other subunits, gathered into LDS.
+  // The neighbour cells this bead's list radius reaches: on each axis the lower (upper) one only
+  // when the radius box crosses the lower (upper) face of the bead's cell (bits k and 3 + k);
+  // records in the other cells fail the radius test anyway (a WCA-only bead, radius ~1.4 sigma
+  // against cells of the largest list radius, needs at most two cells per axis). Only reached
+  // cells are gathered and scanned.
+  int reach = 0;
+  {
+    const float rr = sqrtf(bound2) + 1e-3f, ia = P.inv_a_lj;
+    const float pc[3] = {T.ix, T.iy, T.iz};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float tc = (pc[k] + hbox) * ia, fl = floorf(tc), f = tc - fl;
+      const bool in = fl >= 0.0f && fl < (float)P.g_lj;  // else the cell was clamped: both neighbours
+      if (!in || f < rr * ia) reach |= 1 << k;
+      if (!in || 1.0f - f < rr * ia) reach |= 8 << k;
+    }
+  }
   const bool lj_ok = gather_cells(d.cell_lj, d.cnt_lj, d.ccap_lj, P.c0_lj, P.g_lj, P.inv_a_lj, hbox, cx, cy, cz, active,
-                                  T.ix, T.iy, T.iz, tid, ax, rel, cnum, ownc, coff, rec, red_i, s * np, (s + 1) * np, box);
+                                  T.ix, T.iy, T.iz, tid, ax, rel, cnum, ownc, coff, rec, red_i, s * np, (s + 1) * np, box,
+                                  reach);
   bool direct = !lj_ok;
   if (lj_ok && active) {
     // the local box in registers; with no wrap along x the three cells of a (dz, dy) row are one
@@ -1109,21 +1131,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
     const int ox = fx ? ax[0].cc : -ax[0].lo, oy = fy ? ax[1].cc : -ax[1].lo, oz = fz ? ax[2].cc : -ax[2].lo;
     const bool rows = !fx && g > 1;
     const int nseg = g == 1 ? 1 : (rows ? 9 : 27);
-    // the neighbour cells this bead's list radius reaches: on each axis the lower (upper) one
-    // only when the radius box crosses the lower (upper) face of the bead's cell; records in the
-    // other cells fail the radius test anyway (a WCA-only bead, radius ~1.4 sigma against cells of
-    // the largest list radius, needs at most two cells per axis)
-    const float rr = sqrtf(bound2) + 1e-3f, ia = P.inv_a_lj;
     bool lo[3], hi[3];
-    {
-      const float pc[3] = {T.ix, T.iy, T.iz};
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        const float tc = (pc[k] + hbox) * ia, fl = floorf(tc), f = tc - fl;
-        const bool in = fl >= 0.0f && fl < (float)g;  // else the cell was clamped: both neighbours
-        lo[k] = !in || f < rr * ia;
-        hi[k] = !in || 1.0f - f < rr * ia;
-      }
+    for (int k = 0; k < 3; k++) {
+      lo[k] = (reach >> k) & 1;
+      hi[k] = (reach >> (3 + k)) & 1;
     }
     // the bead in the records' unwrapped frame; no record then needs an image shift unless the
     // local box covers an axis whole or spans half of it
