@@ -220,6 +220,8 @@ def main():
     ap.add_argument("--rep-steps", type=int, default=500, help="timed steps of the replica summary in the md line")
     ap.add_argument("--tune-steps", type=int, default=100)
     ap.add_argument("--graph-steps", type=int, default=100, help="steps per graph launch of the md workload")
+    ap.add_argument("--integrator", choices=["md", "bd"], default="md",
+                    help="integrator of the md workload's main line (bd: Brownian dynamics, for its kernel traces)")
     ap.add_argument("--rep-one-launch", type=int, default=1,
                     help="replica batch: the timed steps as one graph launch (0: 100-step launches)")
     ap.add_argument("--cpu-replicas", type=int, default=8)
@@ -572,7 +574,8 @@ def replica_summary(args):
 
 
 def bench_md(args):
-    md = run_gpu(args)
+    bd_main = args.integrator == "bd"
+    md = run_gpu(args, brownian=bd_main)
     st = md["stats"]
     t_force = st["k_force_ms"]
     out = {
@@ -588,7 +591,8 @@ def bench_md(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"trimer_MVM x{args.subunits} plain FP64 Nose-Hoover MD (q=5, Q=10), "
+        "config": {"workload": f"trimer_MVM x{args.subunits} plain FP64 "
+                               + ("Brownian dynamics (-D b -d 100), " if bd_main else "Nose-Hoover MD (q=5, Q=10), ") +
                                f"C=800uM c=100mM E=2.1 dt=0.005 K=298 ks=50 kb=20, skin {args.skin}",
                    "subunits": args.subunits, "beads": md["nb"], "tile": md["tile"], "parallelism": "single GPU"},
         "roofline": roofline_entry(md["nb"], md["n_list"], md["k_rebuild"], t_force, md["value"],
