@@ -1236,33 +1236,50 @@ __global__ __launch_bounds__(kBlock) void k_cell_query(Dev d, int force_rebuild)
     const float rq2 = rq * rq;
     const int nql = min(nq, kCqQMax);
     const int ncell = g == 1 ? 1 : 27;
-    for (int it = tid; it < nql * ncell; it += kBlock) {
-      const int a = it / ncell, k = it - a * ncell;
-      int c = P.c0_q;
-      if (g > 1)
-        c += (wrap_cell(qcell[a][2] + k / 9 - 1, g) * g + wrap_cell(qcell[a][1] + (k / 3) % 3 - 1, g)) * g +
-             wrap_cell(qcell[a][0] + k % 3 - 1, g);
-      const float4 qa = qsh[a];
-      const int self = __float_as_int(qa.w);
-      const int n = min(d.cnt_q[c], d.ccap_q);
-      const float4* cell = d.cell_q + (size_t)c * d.ccap_q;
-      for (int m0 = 0; m0 < n; m0 += 8) {  // eight records requested together
-        float4 e[8];
+    // with five or more cells per axis the stencil's records all lie less than half a box from the
+    // bead along each axis once the stencil cell's periodic image is applied to the bead (one box
+    // when the cell index wrapped): plain differences then; smaller grids take the minimum image
+    auto yscan = [&](auto wrap_tag) {
+      constexpr bool W = decltype(wrap_tag)::value;
+      for (int it = tid; it < nql * ncell; it += kBlock) {
+        const int a = it / ncell, k = it - a * ncell;
+        int c = P.c0_q;
+        float4 qa = qsh[a];
+        if (g > 1) {
+          const int ux = qcell[a][0] + k % 3 - 1, uy = qcell[a][1] + (k / 3) % 3 - 1, uz = qcell[a][2] + k / 9 - 1;
+          c += (wrap_cell(uz, g) * g + wrap_cell(uy, g)) * g + wrap_cell(ux, g);
+          if (!W) {  // the bead's image next to the cell's records
+            qa.x += ux < 0 ? box : (ux >= g ? -box : 0.0f);
+            qa.y += uy < 0 ? box : (uy >= g ? -box : 0.0f);
+            qa.z += uz < 0 ? box : (uz >= g ? -box : 0.0f);
+          }
+        }
+        const int self = __float_as_int(qa.w);
+        const int n = min(d.cnt_q[c], d.ccap_q);
+        const float4* cell = d.cell_q + (size_t)c * d.ccap_q;
+        for (int m0 = 0; m0 < n; m0 += 8) {  // eight records requested together
+          float4 e[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++)
-          if (m0 + u < n) e[u] = cell[m0 + u];
+          for (int u = 0; u < 8; u++)
+            if (m0 + u < n) e[u] = cell[m0 + u];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-          if (m0 + u >= n) break;
-          const int j = __float_as_int(e[u].w);
-          const float dx = min_imgf_near(qa.x - e[u].x, box, T.ibox), dy = min_imgf_near(qa.y - e[u].y, box, T.ibox),
-                      dz = min_imgf_near(qa.z - e[u].z, box, T.ibox);
-          if (j == self || !(dx * dx + dy * dy + dz * dz < rq2)) continue;
-          const int slot = atomicAdd(&qn[a], 1);
-          if (slot < kCqQSlots) qbuf[a * kCqQSlots + slot] = j;
+          for (int u = 0; u < 8; u++) {
+            if (m0 + u >= n) break;
+            const int j = __float_as_int(e[u].w);
+            float dx = qa.x - e[u].x, dy = qa.y - e[u].y, dz = qa.z - e[u].z;
+            if (W) {
+              dx = min_imgf_near(dx, box, T.ibox);
+              dy = min_imgf_near(dy, box, T.ibox);
+              dz = min_imgf_near(dz, box, T.ibox);
+            }
+            if (j == self || !(dx * dx + dy * dy + dz * dz < rq2)) continue;
+            const int slot = atomicAdd(&qn[a], 1);
+            if (slot < kCqQSlots) qbuf[a * kCqQSlots + slot] = j;
+          }
         }
       }
-    }
+    };
+    if (g >= 5) yscan(Flag<false>{}); else yscan(Flag<true>{});
     __syncthreads();
     // rank sort: an entry's rank is the number of smaller ids in its list; the entries of all
     // lists are spread over the threads (lists that overflowed their slots are left to the
