@@ -856,6 +856,19 @@ int orc_set_thermostat(void* hv, const double* xi, const double* eta, int n, dou
   });
 }
 
+// Test knob: a neighbour-list capacity per bead other than the reference's fixed
+// ceil((NListCutoff + 0.5)^3) (src/md.cpp:231), for states denser than that capacity was sized
+// for (the reference would overflow it there); the list algorithm itself is unchanged. Call
+// before orc_set_state.
+int orc_set_list_capacity(void* hv, int nlv) {
+  return dispatch((OrcHandle*)hv, [&](auto& s) {
+    if (nlv <= 0) return -1;
+    s.NLV = (unsigned)std::min(nlv, s.N);
+    s.itsN.assign(s.N, std::vector<int>(s.NLV, -1));
+    return 0;
+  });
+}
+
 int orc_get_state(void* hv, double* pos, double* vel, double* xi, double* eta) {
   return dispatch((OrcHandle*)hv, [&](auto& s) {
     for (int i = 0; i < s.N; i++) {

@@ -27,6 +27,7 @@ def load():
         lib.orc_set_state.argtypes = [ctypes.c_void_p, _D, _D]
         lib.orc_get_state.argtypes = [ctypes.c_void_p, _D, _D, _D, _D]
         lib.orc_set_thermostat.argtypes = [ctypes.c_void_p, _D, _D, ctypes.c_int, ctypes.c_double]
+        lib.orc_set_list_capacity.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.orc_forces.argtypes = [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D, _D]
         lib.orc_energies.argtypes = [ctypes.c_void_p, _D, _D]
         lib.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int]
@@ -71,6 +72,13 @@ class Oracle:
         (self.box, self.ecut, self.lb, self.kappa, self.ecut_el, self.rc_list, self.T, self.si_time) = p[:8]
         self.nlv, self.N, self.Ns, self.np_ = int(p[8]), int(p[9]), int(p[10]), int(p[11])
         self.step_index = 0
+
+    def set_list_capacity(self, nlv):
+        """Neighbour-list slots per bead (the reference's fixed capacity is sized for its usual
+        densities); call before set_state."""
+        if load().orc_set_list_capacity(self.ptr, int(nlv)) != 0:
+            raise RuntimeError("bad list capacity")
+        self.nlv = min(int(nlv), self.N)
 
     def centre_local(self):
         p = np.zeros(13)
